@@ -1,0 +1,177 @@
+/*
+ * pcw.h -- C ABI of the MI355X (gfx950) wave-propagation engine.
+ *
+ * One handle holds one or more independent 2D shallow-water domains
+ * ("instances"), each a frame of nx*ny interior cells with two ghost layers,
+ * resident in HBM as structure-of-arrays planes h, hu, hv.  All entry points
+ * are plain C: pointers, sizes and status codes, no C++ or torch types.
+ *
+ * The entry points replace the reference's per-step path behind its Fortran
+ * module interfaces (paths relative to the paraCLAWs repository):
+ *   pcw_step + pcw_accept/pcw_reject
+ *       -> solve_dimensional_split (src/solver/MOD_solve.f90:23-175) with the
+ *          copy/rollback of src/main.f90:201,221; the fused step also applies
+ *          integrate_source_term (MOD_solve.f90:295-336) and the physical BC
+ *          fill of update_rank_boundaries
+ *          (src/user_subroutines/SMOD_boundary_conditions.f90:17-213)
+ *   pcw_sweep            -> one sweep of solve_dimensional_split (:71-105, :124-157)
+ *   pcw_fill_ghosts      -> update_rank_boundaries (:17-56) on one rank
+ *   pcw_source           -> integrate_source_term (MOD_solve.f90:295-336)
+ *   pcw_set_state/get_state -> solution%vars(3,ilow-2:ihigh+2,jlow-2:jhigh+2)
+ *                          (src/domain/SMOD_domain_allocation.f90:231)
+ *   pcw_run              -> the time loop of src/main.f90:193-283 with the
+ *                          controller of src/time_control/MOD_time_control.f90
+ *                          replicated on the device
+ *
+ * Return codes: 0 ok, 1 step rejected (solve_dimensional_split's ierr = -1),
+ * negative = fatal (the host maps it to abort_program,
+ * src/utility/MOD_error_handling.f90:122-135); pcw_last_error() describes it.
+ */
+#ifndef PCW_H
+#define PCW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCW_OK 0
+#define PCW_REJECTED 1
+#define PCW_ERR_ARG (-1)
+#define PCW_ERR_HIP (-2)
+#define PCW_ERR_STATE (-3)   /* NaN Jacobian / complex eigenvalues (reference error stop) */
+#define PCW_ERR_BC (-4)      /* BC code 1 (user) or unknown: abort_program */
+#define PCW_ERR_CONTROL (-5) /* end_program reached by the time controller */
+
+#define PCW_FP64 0           /* reference arithmetic order, bit-exact with the FP64 CPU restatement */
+#define PCW_FP32 1           /* FP32 state and flux arithmetic, FP64 time/dt/totals */
+
+#define PCW_ENGINE_FUSED 0   /* one LDS-free register-streaming kernel per step */
+#define PCW_ENGINE_STAGED 1  /* FP64 only: one kernel per reference stage, HBM work arrays */
+
+#define PCW_BC_ZERO_GRADIENT 2
+#define PCW_BC_CONSTANT_GRADIENT 3
+#define PCW_BC_REFLECTIVE 4  /* wall: mirror two interior cells, negate normal momentum (no reference counterpart) */
+
+#define PCW_SRC_NONE 0
+#define PCW_SRC_BATHY 1
+#define PCW_SRC_FRICTION 2
+#define PCW_SRC_BATHY_FRICTION 3
+
+#define PCW_STATUS_RUNNING 0
+#define PCW_STATUS_DONE 1
+#define PCW_STATUS_ERROR 2
+
+typedef struct pcw_handle pcw_handle;
+
+/* Per-instance numerical settings; field meaning follows control.in
+ * (src/utility/MOD_IO.f90:61-94) and MOD_solver_parameters. */
+typedef struct pcw_domain {
+  int32_t nx, ny;              /* m_xi, m_eta (interior cells)                 */
+  int32_t bc[4];               /* bc_xi_lower, bc_xi_upper, bc_eta_lower, bc_eta_upper */
+  int32_t source;              /* PCW_SRC_*                                     */
+  int32_t pad_;
+  double xi_min, xi_max, eta_min, eta_max;
+  double manning_n;            /* per-instance Manning coefficient             */
+} pcw_domain;
+
+/* Settings shared by all instances of a handle. */
+typedef struct pcw_params {
+  int32_t precision;           /* PCW_FP64 / PCW_FP32                           */
+  int32_t engine;              /* PCW_ENGINE_FUSED / PCW_ENGINE_STAGED          */
+  int32_t higher_order;        /* add_higher_order_flux_correction              */
+  int32_t use_limiter;         /* use_limiters                                  */
+  int32_t limiter;             /* 1 minmod, 2 superbee, 3 van Leer, 4 MC        */
+  int32_t tile_rows;           /* rows per tile (0 = autotuned/default)         */
+  int32_t waves_per_block;     /* waves per workgroup (0 = default)             */
+  int32_t pad_;
+  double grav;                 /* g (the reference stores REAL*4 9.81 widened)  */
+  double eps;                  /* ud_precision                                  */
+  double dry_tol;              /* wet/dry threshold, 0 = off (reference)        */
+} pcw_params;
+
+/* time_control settings (MOD_time_control.f90:70-151). */
+typedef struct pcw_time {
+  double t0, t_final, dt0, cfl_target, dt_max, dt_min, growth, shrink;
+  int32_t n_snapshots;
+  int32_t adaptive;
+} pcw_time;
+
+/* One attempted step, as recorded by the device controller. */
+typedef struct pcw_step_record {
+  int32_t accepted;
+  int32_t snapshot;            /* snapshot index reached at this step, 0 if none */
+  double dt_used, cfl, time, dt_next;
+} pcw_step_record;
+
+/* Per-instance status after pcw_run. */
+typedef struct pcw_status {
+  int32_t status;              /* PCW_STATUS_*                                  */
+  int32_t error;               /* PCW_ERR_* when status is ERROR                */
+  int64_t accepted, rejected;
+  double time, dt, cfl_last;
+} pcw_status;
+
+/* ---- lifetime ---------------------------------------------------------- */
+int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains,
+               pcw_handle** out);
+int pcw_destroy(pcw_handle* h);
+const char* pcw_last_error(const pcw_handle* h);
+int pcw_device_bytes(const pcw_handle* h, int64_t* bytes);
+
+/* ---- state transfer (Fortran AoS frame <-> device SoA) ------------------
+ * vars points at solution%vars(1,-1,-1): 3*(nx+4)*(ny+4) doubles, equation
+ * index fastest, then i, then j.  FP32 instances round on upload. */
+int pcw_set_state(pcw_handle* h, int32_t inst, const double* vars);
+int pcw_get_state(pcw_handle* h, int32_t inst, double* vars);
+/* bathymetry frame b(-1:nx+2, -1:ny+2), doubles */
+int pcw_set_bathymetry(pcw_handle* h, int32_t inst, const double* b);
+
+/* ---- host-controlled stepping (the Fortran host's path) -----------------
+ * pcw_step advances every instance by one attempt with its own dt into the
+ * pending buffer (fused: both sweeps + source + ghost fill; staged: the
+ * reference stages in place after a backup copy).  smax[2*k], smax[2*k+1]
+ * receive instance k's max wave speed of the xi and eta sweep; the CFL of a
+ * sweep is (smax*dt)/delta, exactly max(lambda*dt/delta) of MOD_solve.f90:81.
+ * The host then calls pcw_accept (keep) or pcw_reject (roll back). */
+int pcw_step(pcw_handle* h, const double* dt, double* smax);
+int pcw_accept(pcw_handle* h);
+int pcw_reject(pcw_handle* h);
+/* Single sweeps in place on the current state (staged engine semantics). */
+int pcw_sweep(pcw_handle* h, int32_t ixy, const double* dt, double* smax);
+int pcw_source(pcw_handle* h, const double* dt);
+int pcw_fill_ghosts(pcw_handle* h);
+
+/* ---- device-controlled run (batched / persistent path) ------------------
+ * Initialises the device replica of time_control for every instance. */
+int pcw_time_init(pcw_handle* h, const pcw_time* t);
+/* Optional per-instance step log (capacity records per instance). */
+int pcw_enable_log(pcw_handle* h, int32_t capacity);
+int pcw_read_log(pcw_handle* h, int32_t inst, pcw_step_record* out, int32_t cap,
+                 int32_t* n);
+/* Runs graph-captured blocks of steps_per_graph attempts until every
+ * instance is done or max_attempts attempts were made (0 = unbounded).
+ * max_accepted > 0 also stops an instance after that many accepted steps. */
+int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts,
+            int64_t max_accepted);
+int pcw_get_status(pcw_handle* h, int32_t inst, pcw_status* out);
+
+/* ---- measurement --------------------------------------------------------- */
+/* FP64 sums over the interior of instance inst: h, hu, hv. */
+int pcw_totals(pcw_handle* h, int32_t inst, double out[3]);
+/* Device time (ms) of the last pcw_run / pcw_step, measured with HIP events
+ * on the handle's stream. */
+int pcw_last_elapsed_ms(pcw_handle* h, double* ms);
+/* Average duration (ms) of one fused step kernel over the last pcw_run,
+ * from HIP events bracketing each graph launch divided by its steps. */
+int pcw_kernel_stats(pcw_handle* h, double* avg_step_ms, int64_t* launches);
+/* Autotuner: times candidate (tile_rows, waves_per_block) on the handle's
+ * current state and keeps the fastest; writes the choice. */
+int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows,
+                 int32_t* waves_per_block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,729 @@
+// pcw_api.hip -- the extern "C" entry points declared in include/pcw.h.
+//
+// Memory: every instance owns two frames (ping-pong) of three SoA planes
+// h, hu, hv of (pitch x (ny+4)) elements, pitch = nx+4 rounded up to 64
+// elements, all in one HBM pool; descriptors (KInst) and the per-instance
+// controller state (InstState) are device resident.  One HIP stream per
+// handle, default priority.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pcw.h"
+#include "pcw_engine.hpp"
+#include "pcw_kernels.hpp"
+
+using namespace pcw;
+
+struct pcw_handle {
+  pcw_params p{};
+  int n = 0;
+  int esize = 8;
+  std::vector<pcw_domain> dom;
+  std::vector<KInst> hinst;
+  std::vector<InstState> hst;
+  std::vector<int4> htiles;
+  KInst* dinst = nullptr;
+  InstState* dst = nullptr;
+  int4* dtiles = nullptr;
+  int ntiles = 0;
+  void* pool = nullptr;
+  long long pool_elems = 0;
+  void* bpool = nullptr;
+  long long bpool_elems = 0;
+  double* dsnaps = nullptr;
+  pcw_step_record* dlog = nullptr;
+  int log_cap = 0;
+  double* work = nullptr;
+  double* dscratch = nullptr;     // 4096 doubles: dt upload, totals partials
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int tile_rows = 64, wpb = 2;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int gsteps = 0;
+  std::string err;
+  double last_ms = 0.0, avg_step_ms = 0.0;
+  long long launches = 0;
+  int max_nx = 0, max_ny = 0, max_side = 0;
+};
+
+namespace {
+
+constexpr int kScratch = 4096;
+
+int fail(pcw_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail(h, PCW_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+void build_tiles(pcw_handle* h, int ty) {
+  h->htiles.clear();
+  for (int k = 0; k < h->n; ++k) {
+    KInst& I = h->hinst[k];
+    std::vector<int> xs, y0s;
+    const int nsx = (I.nx + kStripOut - 1) / kStripOut;
+    for (int s = 0; s < nsx; ++s) {
+      int x0 = 1 + kStripOut * s;
+      if (x0 + kStripOut - 1 > I.nx) x0 = std::max(1, I.nx - kStripOut + 1);
+      xs.push_back(x0);
+    }
+    const int nsy = (I.ny + ty - 1) / ty;
+    int count = 0;
+    for (int b = 0; b < nsy; ++b) {
+      int y0 = 1 + ty * b;
+      if (y0 + ty - 1 > I.ny) y0 = std::max(1, I.ny - ty + 1);
+      int y1 = std::min(y0 + ty - 1, I.ny);
+      for (int x0 : xs) {
+        h->htiles.push_back(make_int4(k, x0, y0, y1));
+        ++count;
+      }
+    }
+    I.ntiles = count;
+  }
+  h->ntiles = (int)h->htiles.size();
+}
+
+int upload_tiles(pcw_handle* h) {
+  if (h->dtiles) HIPCHK(hipFree(h->dtiles));
+  h->dtiles = nullptr;
+  HIPCHK(hipMalloc(&h->dtiles, sizeof(int4) * std::max(1, h->ntiles)));
+  HIPCHK(hipMemcpy(h->dtiles, h->htiles.data(), sizeof(int4) * h->ntiles, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->dinst, h->hinst.data(), sizeof(KInst) * h->n, hipMemcpyHostToDevice));
+  return PCW_OK;
+}
+
+int pull_states(pcw_handle* h) {
+  HIPCHK(hipMemcpyAsync(h->hst.data(), h->dst, sizeof(InstState) * h->n, hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PCW_OK;
+}
+
+int push_states(pcw_handle* h) {
+  HIPCHK(hipMemcpyAsync(h->dst, h->hst.data(), sizeof(InstState) * h->n, hipMemcpyHostToDevice,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PCW_OK;
+}
+
+void drop_graph(pcw_handle* h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  h->gsteps = 0;
+}
+
+int limiter_code(const pcw_params& p) { return (p.higher_order && p.use_limiter) ? p.limiter : 0; }
+
+// enqueue one fused attempt for every instance
+int enqueue_fused(pcw_handle* h) {
+  const int threads = 64 * h->wpb;
+  int blocks = (h->ntiles + h->wpb - 1) / h->wpb;
+  blocks = (int)round_up(blocks, 8);
+  hipError_t e;
+  if (h->p.precision == PCW_FP32) {
+    FusedArgs<float> a{(float*)h->pool, (const float*)h->bpool, h->dinst, h->dst, h->dtiles,
+                       h->ntiles, h->p.higher_order, h->p.dry_tol > 0.0, 0, h->p.grav,
+                       h->p.eps, h->p.dry_tol};
+    e = launch_fused_f32(limiter_code(h->p), a, blocks, threads, h->stream);
+  } else {
+    FusedArgs<double> a{(double*)h->pool, (const double*)h->bpool, h->dinst, h->dst, h->dtiles,
+                        h->ntiles, h->p.higher_order, h->p.dry_tol > 0.0, 0, h->p.grav,
+                        h->p.eps, h->p.dry_tol};
+    e = launch_fused_f64(limiter_code(h->p), a, blocks, threads, h->stream);
+  }
+  if (e != hipSuccess) return fail(h, PCW_ERR_HIP, std::string("fused launch: ") + hipGetErrorString(e));
+  return PCW_OK;
+}
+
+// enqueue the staged stage kernels of one attempt (without the finish)
+int enqueue_staged_sweeps(pcw_handle* h, int backup) {
+  const size_t wplane_total = 0;
+  (void)wplane_total;
+  if (backup) {
+    hipError_t e = launch_copy<double>((double*)h->pool, h->dinst, h->dst, h->n, 1, 0, h->stream);
+    if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
+  }
+  long long woff = 0;
+  for (int k = 0; k < h->n; ++k) {
+    const KInst& I = h->hinst[k];
+    StagedArgs a{h->dinst + k, h->dst + k, (double*)h->pool + I.off[0], h->work + woff,
+                 h->p.grav, h->p.eps, h->p.dry_tol, limiter_code(h->p), 0};
+    for (int dir = 1; dir <= 2; ++dir) {
+      hipError_t e = launch_staged_sweep(a, I, dir, h->p.higher_order, h->stream);
+      if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
+    }
+    woff += (long long)W_COUNT * I.plane;
+  }
+  return PCW_OK;
+}
+
+template <typename T>
+int enqueue_source_ghosts(pcw_handle* h) {
+  hipError_t e;
+  bool any_src = h->p.dry_tol > 0.0;
+  for (const auto& d : h->dom) any_src = any_src || d.source != PCW_SRC_NONE;
+  if (any_src) {
+    e = launch_source<T>((T*)h->pool, (const T*)h->bpool, h->dinst, h->dst, h->n, h->max_nx,
+                         h->max_ny, h->p.dry_tol > 0.0, h->p.grav, h->p.dry_tol, h->stream);
+    if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
+  }
+  e = launch_ghosts<T>((T*)h->pool, h->dinst, h->dst, h->n, h->max_side, -1, h->stream);
+  if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
+  return PCW_OK;
+}
+
+// one device-controlled attempt (graph body)
+int enqueue_attempt(pcw_handle* h) {
+  if (h->p.engine == PCW_ENGINE_FUSED) return enqueue_fused(h);
+  int rc = enqueue_staged_sweeps(h, 1);
+  if (rc) return rc;
+  rc = enqueue_source_ghosts<double>(h);
+  if (rc) return rc;
+  hipError_t e = launch_st_finish(h->dinst, h->dst, h->n, h->stream);
+  if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
+  e = launch_copy<double>((double*)h->pool, h->dinst, h->dst, h->n, 0, 1, h->stream);
+  if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
+  return PCW_OK;
+}
+
+int set_dts(pcw_handle* h, const double* dt, int controlled) {
+  if (h->n > kScratch) return fail(h, PCW_ERR_ARG, "too many instances for host control");
+  HIPCHK(hipMemcpyAsync(h->dscratch, dt, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(launch_set_dt(h->dst, h->dscratch, h->n, controlled, h->stream));
+  return PCW_OK;
+}
+
+int check_errors(pcw_handle* h) {
+  for (int k = 0; k < h->n; ++k)
+    if (h->hst[k].status == PCW_STATUS_ERROR) {
+      int e = h->hst[k].error ? h->hst[k].error : PCW_ERR_STATE;
+      return fail(h, e,
+                  e == PCW_ERR_STATE
+                      ? "instance " + std::to_string(k) +
+                            ": NaN in the Jacobian or complex eigenvalues (h <= 0)"
+                      : "instance " + std::to_string(k) + ": time controller reached end_program");
+    }
+  return PCW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pcw_last_error(const pcw_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains,
+               pcw_handle** out) {
+  if (!p || !domains || n_domains < 1 || !out) return PCW_ERR_ARG;
+  *out = nullptr;
+  pcw_handle* h = new pcw_handle();
+  h->p = *p;
+  h->n = n_domains;
+  h->esize = (p->precision == PCW_FP32) ? 4 : 8;
+  if (p->precision != PCW_FP32 && p->precision != PCW_FP64) {
+    fail(h, PCW_ERR_ARG, "precision must be PCW_FP64 or PCW_FP32");
+    *out = h;
+    return PCW_ERR_ARG;
+  }
+  if (p->engine == PCW_ENGINE_STAGED && p->precision != PCW_FP64) {
+    fail(h, PCW_ERR_ARG, "the staged engine is FP64 only");
+    *out = h;
+    return PCW_ERR_ARG;
+  }
+  if (p->higher_order && p->use_limiter && (p->limiter < 1 || p->limiter > 4)) {
+    fail(h, PCW_ERR_ARG, "invalid limiter choice => ABORTING");
+    *out = h;
+    return PCW_ERR_ARG;
+  }
+  h->dom.assign(domains, domains + n_domains);
+  h->hinst.resize(n_domains);
+  h->hst.assign(n_domains, InstState{});
+  bool any_bathy = false;
+  long long base = 0, bbase = 0;
+  for (int k = 0; k < n_domains; ++k) {
+    const pcw_domain& d = domains[k];
+    if (d.nx < 2 || d.ny < 2) {
+      fail(h, PCW_ERR_ARG, "each domain needs nx >= 2 and ny >= 2");
+      *out = h;
+      return PCW_ERR_ARG;
+    }
+    for (int s = 0; s < 4; ++s)
+      if (d.bc[s] < PCW_BC_ZERO_GRADIENT || d.bc[s] > PCW_BC_REFLECTIVE) {
+        fail(h, PCW_ERR_BC,
+             d.bc[s] == 1 ? "User-defined boundary condition not implemented yet!"
+                          : "boundary condition not implemented yet!");
+        *out = h;
+        return PCW_ERR_BC;
+      }
+    KInst& I = h->hinst[k];
+    std::memset(&I, 0, sizeof I);
+    I.nx = d.nx;
+    I.ny = d.ny;
+    I.pitch = (int)round_up(d.nx + 4, 64);
+    I.plane = (long long)I.pitch * (d.ny + 4);
+    I.off[0] = base;
+    I.off[1] = base + 3 * I.plane;
+    base += 6 * I.plane;
+    for (int s = 0; s < 4; ++s) I.bc[s] = d.bc[s];
+    I.source = d.source;
+    I.dxi = (d.xi_max - d.xi_min) / d.nx;   // SMOD_mesh_manipulation.f90:28-29
+    I.deta = (d.eta_max - d.eta_min) / d.ny;
+    I.xi_min = d.xi_min;
+    I.eta_min = d.eta_min;
+    I.manning = d.manning_n;
+    if (d.source == PCW_SRC_BATHY || d.source == PCW_SRC_BATHY_FRICTION) {
+      I.boff = bbase;
+      bbase += I.plane;
+      any_bathy = true;
+    } else {
+      I.boff = -1;
+    }
+    h->max_nx = std::max(h->max_nx, d.nx);
+    h->max_ny = std::max(h->max_ny, d.ny);
+    h->max_side = std::max(h->max_side, d.nx + d.ny);
+    InstState& S = h->hst[k];
+    std::memset(&S, 0, sizeof S);
+    S.status = PCW_STATUS_RUNNING;
+    S.last_ok = 1;
+  }
+  *out = h;
+  h->pool_elems = base;
+  h->bpool_elems = bbase;
+  h->tile_rows = p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 32);
+  h->wpb = (p->waves_per_block >= 1 && p->waves_per_block <= 4) ? p->waves_per_block : 2;
+  HIPCHK(hipStreamCreate(&h->stream));
+  HIPCHK(hipEventCreate(&h->ev0));
+  HIPCHK(hipEventCreate(&h->ev1));
+  HIPCHK(hipMalloc(&h->pool, (size_t)base * h->esize));
+  HIPCHK(hipMemset(h->pool, 0, (size_t)base * h->esize));
+  if (any_bathy) {
+    HIPCHK(hipMalloc(&h->bpool, (size_t)bbase * h->esize));
+    HIPCHK(hipMemset(h->bpool, 0, (size_t)bbase * h->esize));
+  }
+  HIPCHK(hipMalloc(&h->dinst, sizeof(KInst) * n_domains));
+  HIPCHK(hipMalloc(&h->dst, sizeof(InstState) * n_domains));
+  HIPCHK(hipMalloc(&h->dscratch, sizeof(double) * kScratch));
+  if (p->engine == PCW_ENGINE_STAGED) {
+    long long welems = 0;
+    for (const auto& I : h->hinst) welems += (long long)W_COUNT * I.plane;
+    HIPCHK(hipMalloc(&h->work, (size_t)welems * sizeof(double)));
+    HIPCHK(hipMemset(h->work, 0, (size_t)welems * sizeof(double)));
+  }
+  build_tiles(h, h->tile_rows);
+  int rc = upload_tiles(h);
+  if (rc) return rc;
+  return push_states(h);
+}
+
+int pcw_destroy(pcw_handle* h) {
+  if (!h) return PCW_ERR_ARG;
+  drop_graph(h);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  void* ptrs[] = {h->pool, h->bpool, h->dinst, h->dst, h->dtiles, h->dsnaps, h->dlog, h->work,
+                  h->dscratch};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return PCW_OK;
+}
+
+int pcw_device_bytes(const pcw_handle* h, int64_t* bytes) {
+  if (!h || !bytes) return PCW_ERR_ARG;
+  long long w = 0;
+  if (h->work)
+    for (const auto& I : h->hinst) w += (long long)W_COUNT * I.plane * 8;
+  *bytes = h->pool_elems * h->esize + h->bpool_elems * h->esize + w;
+  return PCW_OK;
+}
+
+// ------------------------------------------------------------ state I/O
+int pcw_set_state(pcw_handle* h, int32_t inst, const double* vars) {
+  if (!h || !vars || inst < 0 || inst >= h->n) return fail(h, PCW_ERR_ARG, "bad argument");
+  const KInst& I = h->hinst[inst];
+  const int nxf = I.nx + 4, nyf = I.ny + 4;
+  std::vector<char> buf((size_t)3 * I.plane * h->esize, 0);
+  for (int j = 0; j < nyf; ++j)
+    for (int i = 0; i < nxf; ++i)
+      for (int me = 0; me < 3; ++me) {
+        double v = vars[me + 3 * ((size_t)i + (size_t)nxf * j)];
+        size_t o = (size_t)me * I.plane + (size_t)j * I.pitch + i;
+        if (h->esize == 4) ((float*)buf.data())[o] = (float)v;
+        else ((double*)buf.data())[o] = v;
+      }
+  for (int b = 0; b < 2; ++b)
+    HIPCHK(hipMemcpy((char*)h->pool + (size_t)I.off[b] * h->esize, buf.data(), buf.size(),
+                     hipMemcpyHostToDevice));
+  return PCW_OK;
+}
+
+int pcw_get_state(pcw_handle* h, int32_t inst, double* vars) {
+  if (!h || !vars || inst < 0 || inst >= h->n) return fail(h, PCW_ERR_ARG, "bad argument");
+  int rc = pull_states(h);
+  if (rc) return rc;
+  const KInst& I = h->hinst[inst];
+  const int cur = h->hst[inst].cur;
+  const int nxf = I.nx + 4, nyf = I.ny + 4;
+  std::vector<char> buf((size_t)3 * I.plane * h->esize);
+  HIPCHK(hipMemcpy(buf.data(), (char*)h->pool + (size_t)I.off[cur] * h->esize, buf.size(),
+                   hipMemcpyDeviceToHost));
+  for (int j = 0; j < nyf; ++j)
+    for (int i = 0; i < nxf; ++i)
+      for (int me = 0; me < 3; ++me) {
+        size_t o = (size_t)me * I.plane + (size_t)j * I.pitch + i;
+        vars[me + 3 * ((size_t)i + (size_t)nxf * j)] =
+            h->esize == 4 ? (double)((float*)buf.data())[o] : ((double*)buf.data())[o];
+      }
+  return PCW_OK;
+}
+
+int pcw_set_bathymetry(pcw_handle* h, int32_t inst, const double* b) {
+  if (!h || !b || inst < 0 || inst >= h->n) return fail(h, PCW_ERR_ARG, "bad argument");
+  const KInst& I = h->hinst[inst];
+  if (I.boff < 0) return fail(h, PCW_ERR_ARG, "instance has no bathymetry source");
+  const int nxf = I.nx + 4, nyf = I.ny + 4;
+  std::vector<char> buf((size_t)I.plane * h->esize, 0);
+  for (int j = 0; j < nyf; ++j)
+    for (int i = 0; i < nxf; ++i) {
+      double v = b[(size_t)i + (size_t)nxf * j];
+      size_t o = (size_t)j * I.pitch + i;
+      if (h->esize == 4) ((float*)buf.data())[o] = (float)v;
+      else ((double*)buf.data())[o] = v;
+    }
+  HIPCHK(hipMemcpy((char*)h->bpool + (size_t)I.boff * h->esize, buf.data(), buf.size(),
+                   hipMemcpyHostToDevice));
+  return PCW_OK;
+}
+
+// ------------------------------------------------- host-controlled stepping
+int pcw_step(pcw_handle* h, const double* dt, double* smax) {
+  if (!h || !dt) return fail(h, PCW_ERR_ARG, "bad argument");
+  drop_graph(h);
+  int rc = set_dts(h, dt, 0);
+  if (rc) return rc;
+  if (h->p.engine == PCW_ENGINE_FUSED) {
+    rc = enqueue_fused(h);
+  } else {
+    rc = enqueue_staged_sweeps(h, 1);
+    if (!rc) rc = enqueue_source_ghosts<double>(h);
+    if (!rc) HIPCHK(launch_st_finish(h->dinst, h->dst, h->n, h->stream));
+  }
+  if (rc) return rc;
+  rc = pull_states(h);
+  if (rc) return rc;
+  rc = check_errors(h);
+  if (rc) return rc;
+  if (smax)
+    for (int k = 0; k < h->n; ++k) {
+      smax[2 * k] = h->hst[k].last_smax_x;
+      smax[2 * k + 1] = h->hst[k].last_smax_y;
+    }
+  return PCW_OK;
+}
+
+int pcw_accept(pcw_handle* h) {
+  if (!h) return PCW_ERR_ARG;
+  if (h->p.engine == PCW_ENGINE_FUSED) HIPCHK(launch_flip(h->dst, h->n, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PCW_OK;
+}
+
+int pcw_reject(pcw_handle* h) {
+  if (!h) return PCW_ERR_ARG;
+  if (h->p.engine == PCW_ENGINE_STAGED)
+    HIPCHK(launch_copy<double>((double*)h->pool, h->dinst, h->dst, h->n, 0, 0, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PCW_OK;
+}
+
+int pcw_sweep(pcw_handle* h, int32_t ixy, const double* dt, double* smax) {
+  if (!h || !dt || (ixy != 1 && ixy != 2)) return fail(h, PCW_ERR_ARG, "bad argument");
+  if (h->p.engine != PCW_ENGINE_STAGED)
+    return fail(h, PCW_ERR_ARG, "single sweeps need the staged engine");
+  drop_graph(h);
+  int rc = set_dts(h, dt, 0);
+  if (rc) return rc;
+  long long woff = 0;
+  for (int k = 0; k < h->n; ++k) {
+    const KInst& I = h->hinst[k];
+    StagedArgs a{h->dinst + k, h->dst + k, (double*)h->pool + I.off[0], h->work + woff,
+                 h->p.grav, h->p.eps, h->p.dry_tol, limiter_code(h->p), 0};
+    HIPCHK(launch_staged_sweep(a, I, ixy, h->p.higher_order, h->stream));
+    woff += (long long)W_COUNT * I.plane;
+  }
+  HIPCHK(launch_st_finish(h->dinst, h->dst, h->n, h->stream));
+  rc = pull_states(h);
+  if (rc) return rc;
+  rc = check_errors(h);
+  if (rc) return rc;
+  if (smax)
+    for (int k = 0; k < h->n; ++k) smax[k] = ixy == 1 ? h->hst[k].last_smax_x : h->hst[k].last_smax_y;
+  return PCW_OK;
+}
+
+int pcw_source(pcw_handle* h, const double* dt) {
+  if (!h || !dt) return fail(h, PCW_ERR_ARG, "bad argument");
+  drop_graph(h);
+  int rc = set_dts(h, dt, 0);
+  if (rc) return rc;
+  hipError_t e;
+  if (h->esize == 4)
+    e = launch_source<float>((float*)h->pool, (const float*)h->bpool, h->dinst, h->dst, h->n,
+                             h->max_nx, h->max_ny, h->p.dry_tol > 0.0, h->p.grav, h->p.dry_tol,
+                             h->stream);
+  else
+    e = launch_source<double>((double*)h->pool, (const double*)h->bpool, h->dinst, h->dst, h->n,
+                              h->max_nx, h->max_ny, h->p.dry_tol > 0.0, h->p.grav, h->p.dry_tol,
+                              h->stream);
+  if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PCW_OK;
+}
+
+int pcw_fill_ghosts(pcw_handle* h) {
+  if (!h) return PCW_ERR_ARG;
+  hipError_t e = h->esize == 4
+                     ? launch_ghosts<float>((float*)h->pool, h->dinst, h->dst, h->n, h->max_side, -1, h->stream)
+                     : launch_ghosts<double>((double*)h->pool, h->dinst, h->dst, h->n, h->max_side, -1, h->stream);
+  if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PCW_OK;
+}
+
+// -------------------------------------------------- device-controlled run
+int pcw_time_init(pcw_handle* h, const pcw_time* t) {
+  if (!h || !t) return PCW_ERR_ARG;
+  if (t->dt_min > t->dt_max) return fail(h, PCW_ERR_CONTROL, "ABORT: Invalid timestep bounds!");
+  drop_graph(h);
+  int rc = pull_states(h);
+  if (rc) return rc;
+  const int ns = t->n_snapshots < 1 ? 1 : t->n_snapshots;
+  if (h->dsnaps) HIPCHK(hipFree(h->dsnaps));
+  h->dsnaps = nullptr;
+  HIPCHK(hipMalloc(&h->dsnaps, sizeof(double) * ns));
+  std::vector<double> snaps(ns);
+  tc_snapshots(*t, snaps.data());
+  HIPCHK(hipMemcpy(h->dsnaps, snaps.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+  for (int k = 0; k < h->n; ++k) {
+    InstState& S = h->hst[k];
+    tc_init(S.tc, *t);
+    S.tc.snaps = h->dsnaps;
+    S.smax_x = S.smax_y = 0ull;
+    S.tiles_done = 0u;
+    S.accepted = S.rejected = 0;
+    S.max_accepted = 0;
+    S.status = PCW_STATUS_RUNNING;
+    S.error = 0;
+    S.bad = 0;
+    S.controlled = 1;
+    S.last_ok = 1;
+    S.log_n = 0;
+  }
+  return push_states(h);
+}
+
+int pcw_enable_log(pcw_handle* h, int32_t capacity) {
+  if (!h || capacity < 0) return PCW_ERR_ARG;
+  drop_graph(h);
+  int rc = pull_states(h);
+  if (rc) return rc;
+  if (h->dlog) HIPCHK(hipFree(h->dlog));
+  h->dlog = nullptr;
+  h->log_cap = capacity;
+  if (capacity > 0)
+    HIPCHK(hipMalloc(&h->dlog, sizeof(pcw_step_record) * (size_t)capacity * h->n));
+  for (int k = 0; k < h->n; ++k) {
+    h->hst[k].log = capacity > 0 ? h->dlog + (size_t)k * capacity : nullptr;
+    h->hst[k].log_cap = capacity;
+    h->hst[k].log_n = 0;
+  }
+  return push_states(h);
+}
+
+int pcw_read_log(pcw_handle* h, int32_t inst, pcw_step_record* out, int32_t cap, int32_t* n) {
+  if (!h || !out || !n || inst < 0 || inst >= h->n) return PCW_ERR_ARG;
+  int rc = pull_states(h);
+  if (rc) return rc;
+  int m = std::min(cap, h->hst[inst].log_n);
+  if (m > 0)
+    HIPCHK(hipMemcpy(out, h->dlog + (size_t)inst * h->log_cap, sizeof(pcw_step_record) * m,
+                     hipMemcpyDeviceToHost));
+  *n = m;
+  return PCW_OK;
+}
+
+int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts, int64_t max_accepted) {
+  if (!h || steps_per_graph < 1) return PCW_ERR_ARG;
+  int rc = pull_states(h);
+  if (rc) return rc;
+  for (int k = 0; k < h->n; ++k) {
+    if (!h->hst[k].controlled) return fail(h, PCW_ERR_ARG, "call pcw_time_init before pcw_run");
+    h->hst[k].max_accepted = max_accepted;
+  }
+  rc = push_states(h);
+  if (rc) return rc;
+  if (!h->gexec || h->gsteps != steps_per_graph) {
+    drop_graph(h);
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < steps_per_graph; ++s) {
+      rc = enqueue_attempt(h);
+      if (rc) {
+        hipGraph_t g;
+        (void)hipStreamEndCapture(h->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+    }
+    HIPCHK(hipStreamEndCapture(h->stream, &h->graph));
+    HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    h->gsteps = steps_per_graph;
+  }
+  long long attempts = 0;
+  double graph_ms = 0.0;
+  long long before = 0;
+  for (int k = 0; k < h->n; ++k)
+    before = std::max(before, h->hst[k].accepted + h->hst[k].rejected);
+  h->launches = 0;
+  for (;;) {
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    rc = pull_states(h);
+    if (rc) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    graph_ms += ms;
+    h->launches += steps_per_graph;
+    attempts += steps_per_graph;
+    bool running = false;
+    for (int k = 0; k < h->n; ++k) running = running || h->hst[k].status == PCW_STATUS_RUNNING;
+    if (!running) break;
+    if (max_attempts > 0 && attempts >= max_attempts) break;
+  }
+  long long after = 0;
+  for (int k = 0; k < h->n; ++k)
+    after = std::max(after, h->hst[k].accepted + h->hst[k].rejected);
+  h->last_ms = graph_ms;
+  h->avg_step_ms = (after > before) ? graph_ms / (double)(after - before) : 0.0;
+  return check_errors(h);
+}
+
+int pcw_get_status(pcw_handle* h, int32_t inst, pcw_status* out) {
+  if (!h || !out || inst < 0 || inst >= h->n) return PCW_ERR_ARG;
+  int rc = pull_states(h);
+  if (rc) return rc;
+  const InstState& S = h->hst[inst];
+  out->status = S.status;
+  out->error = S.error;
+  out->accepted = S.accepted;
+  out->rejected = S.rejected;
+  out->time = S.tc.time;
+  out->dt = S.tc.dt;
+  out->cfl_last = S.cfl_last;
+  return PCW_OK;
+}
+
+// ------------------------------------------------------------ measurement
+int pcw_totals(pcw_handle* h, int32_t inst, double out[3]) {
+  if (!h || !out || inst < 0 || inst >= h->n) return PCW_ERR_ARG;
+  const int nparts = std::min(1024, h->hinst[inst].ny);
+  hipError_t e = h->esize == 4
+                     ? launch_totals<float>((const float*)h->pool, h->dinst + inst, h->dst + inst,
+                                            h->dscratch, nparts, h->dscratch + 3 * 1024, h->stream)
+                     : launch_totals<double>((const double*)h->pool, h->dinst + inst, h->dst + inst,
+                                             h->dscratch, nparts, h->dscratch + 3 * 1024, h->stream);
+  if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
+  HIPCHK(hipMemcpyAsync(out, h->dscratch + 3 * 1024, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PCW_OK;
+}
+
+int pcw_last_elapsed_ms(pcw_handle* h, double* ms) {
+  if (!h || !ms) return PCW_ERR_ARG;
+  *ms = h->last_ms;
+  return PCW_OK;
+}
+
+int pcw_kernel_stats(pcw_handle* h, double* avg_step_ms, int64_t* launches) {
+  if (!h) return PCW_ERR_ARG;
+  if (avg_step_ms) *avg_step_ms = h->avg_step_ms;
+  if (launches) *launches = h->launches;
+  return PCW_OK;
+}
+
+int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* waves_per_block) {
+  if (!h || steps < 1) return PCW_ERR_ARG;
+  if (h->p.engine != PCW_ENGINE_FUSED) return fail(h, PCW_ERR_ARG, "autotuning applies to the fused engine");
+  drop_graph(h);
+  int rc = pull_states(h);
+  if (rc) return rc;
+  std::vector<InstState> saved = h->hst;
+  for (auto& S : h->hst) {
+    S.controlled = 0;
+    S.status = PCW_STATUS_RUNNING;
+  }
+  rc = push_states(h);
+  if (rc) return rc;
+  const int rows_c[] = {16, 32, 64, 128, 256};
+  const int wpb_c[] = {1, 2, 4};
+  double best = 1e30;
+  int br = h->tile_rows, bw = h->wpb;
+  for (int tr : rows_c)
+    for (int wv : wpb_c) {
+      h->wpb = wv;
+      build_tiles(h, tr);
+      rc = upload_tiles(h);
+      if (rc) return rc;
+      rc = enqueue_fused(h);  // warm-up
+      if (rc) return rc;
+      HIPCHK(hipEventRecord(h->ev0, h->stream));
+      for (int s = 0; s < steps; ++s) {
+        rc = enqueue_fused(h);
+        if (rc) return rc;
+      }
+      HIPCHK(hipEventRecord(h->ev1, h->stream));
+      HIPCHK(hipEventSynchronize(h->ev1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+      if (ms < best) {
+        best = ms;
+        br = tr;
+        bw = wv;
+      }
+    }
+  h->tile_rows = br;
+  h->wpb = bw;
+  build_tiles(h, br);
+  rc = upload_tiles(h);
+  if (rc) return rc;
+  h->hst = saved;
+  for (auto& S : h->hst) {
+    S.smax_x = S.smax_y = 0ull;
+    S.tiles_done = 0u;
+  }
+  rc = push_states(h);
+  if (rc) return rc;
+  if (tile_rows) *tile_rows = br;
+  if (waves_per_block) *waves_per_block = bw;
+  return PCW_OK;
+}
+
+}  // extern "C"

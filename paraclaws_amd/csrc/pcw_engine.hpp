@@ -1,0 +1,172 @@
+// pcw_engine.hpp -- instance descriptors, device-resident per-instance state
+// and the FP64 time controller (a replica of time_control's updateTime,
+// src/time_control/MOD_time_control.f90:153-279) that runs both on the host
+// and, for device-controlled runs, in the last workgroup of each step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pcw.h"
+
+namespace pcw {
+
+constexpr int kStripOut = 60;   // output columns per wave (64 lanes - 2x2 halo)
+
+// Read-only description of one instance (domain).
+struct KInst {
+  int nx, ny, pitch, ntiles;
+  int bc[4];
+  int source, pad;
+  long long plane;     // elements per field plane (pitch * (ny+4))
+  long long off[2];    // element offset of the h-plane frame origin (i=-1,j=-1) of buffer 0/1
+  long long boff;      // bathymetry frame offset, -1 when absent
+  double dxi, deta, xi_min, eta_min;
+  double manning;
+};
+
+// time_control fields (MOD_time_control.f90:16-49), FP64.
+struct TimeCtl {
+  double time, time_initial, time_final, dt, dt_previous, dt_max, dt_min;
+  double time_precision, CFL, CFL_target, CFL_previous, shrink, growth, emergency;
+  int adaptive, snapshot_exception, snap_idx, n_snap;
+  int ended;  // end_program reached
+  int pad;
+  const double* snaps;  // snapshot_intervals (device or host pointer)
+};
+
+// Device-resident per-instance state.
+struct InstState {
+  TimeCtl tc;
+  unsigned long long smax_x, smax_y;  // running max of the sweep speeds (double bits)
+  double last_smax_x, last_smax_y;    // maxima of the last completed attempt
+  double cfl_last;
+  long long accepted, rejected;
+  long long max_accepted;             // stop after this many accepted steps (0 = no limit)
+  unsigned int tiles_done;            // completion counter of the running attempt
+  int cur;                            // index of the current buffer
+  int status;                         // PCW_STATUS_*
+  int error;                          // PCW_ERR_* when status is ERROR
+  int bad;                            // set by tiles on a NaN Jacobian / complex eigenvalue
+  int controlled;                     // 1: the last tile runs the controller
+  int last_ok;                        // staged engine: 1 when the last attempt was kept
+  int log_n, log_cap;
+  pcw_step_record* log;
+};
+
+// --------------------------------------------------------- controller
+// initialize, MOD_time_control.f90:70-151 (snapshot table written by the caller)
+__host__ __device__ inline void tc_init(TimeCtl& t, const pcw_time& s) {
+  t.time_initial = s.t0;
+  t.time_final = s.t_final;
+  t.time = s.t0;
+  t.dt = s.dt0;
+  t.dt_previous = s.dt0;
+  t.dt_max = s.dt_max;
+  t.dt_min = s.dt_min;
+  t.adaptive = s.adaptive;
+  t.n_snap = s.n_snapshots < 1 ? 1 : s.n_snapshots;
+  t.snap_idx = 1;
+  t.CFL = 0.0;
+  t.CFL_previous = 0.0;
+  t.CFL_target = s.cfl_target;
+  t.time_precision = 0.01;
+  t.growth = s.growth;
+  t.shrink = s.shrink;
+  t.emergency = 0.25;
+  t.snapshot_exception = 0;
+  t.ended = 0;
+}
+
+// snapshot_intervals, MOD_time_control.f90:133-139
+__host__ __device__ inline void tc_snapshots(const pcw_time& s, double* out) {
+  int n = s.n_snapshots < 1 ? 1 : s.n_snapshots;
+  double interval = (s.t_final - s.t0) / n;
+  for (int i = 1; i <= n; ++i) out[i - 1] = s.t0 + i * interval;
+  out[n - 1] = s.t_final;
+}
+
+// updateTime, MOD_time_control.f90:153-243
+__host__ __device__ inline void tc_update_time(TimeCtl& t, int step_valid) {
+  double change_factor, dt_step;
+  t.dt_previous = t.dt;
+  dt_step = t.dt;
+  if (step_valid) {
+    if (t.adaptive) {
+      if (t.CFL > 0.0) change_factor = t.CFL_target / t.CFL;
+      else change_factor = 1.0;
+      if (change_factor >= t.growth) change_factor = t.growth;
+      else if (fabs(change_factor - 1.0) <= t.time_precision) change_factor = 1.0;
+      dt_step = t.dt_previous * change_factor;
+    }
+    double snap = t.snaps[t.snap_idx - 1];
+    if (t.time + dt_step > snap) {
+      dt_step = snap - t.time;
+      t.snapshot_exception = 1;
+    } else {
+      t.snapshot_exception = 0;
+    }
+  } else {
+    if (!t.adaptive) { t.ended = 1; return; }
+    dt_step = t.emergency * t.dt;
+  }
+  if (dt_step > t.dt_max) dt_step = t.dt_max;
+  if (dt_step < t.dt_min) {
+    if (!t.adaptive && !t.snapshot_exception) { t.ended = 1; return; }
+    dt_step = t.dt_min;
+  }
+  if (step_valid) t.time = t.time + dt_step;
+  t.dt = dt_step;
+}
+
+__host__ __device__ inline void tc_update_max_cfl(TimeCtl& t, double c) {
+  t.CFL = t.CFL > c ? t.CFL : c;
+}
+__host__ __device__ inline int tc_valid(const TimeCtl& t) {
+  return t.CFL_target / t.CFL >= t.shrink;
+}
+__host__ __device__ inline void tc_reset_cfl(TimeCtl& t) {
+  t.CFL_previous = t.CFL;
+  t.CFL = 0.0;
+}
+
+// One attempted step of the main loop (src/main.f90:201-277) given the two
+// sweep maxima: CFL of each sweep as in MOD_solve.f90:81/134, the validity
+// tests after each sweep (:109-120, :161-174), accept/reject, updateTime,
+// snapshot index, reset_CFL.  Returns 1 when accepted.
+__host__ __device__ inline int tc_attempt(TimeCtl& t, double smax_x, double smax_y,
+                                          double dxi, double deta, pcw_step_record* rec) {
+  const double dt = t.dt;
+  double cx = smax_x * dt / dxi;
+  double cy = smax_y * dt / deta;
+  tc_update_max_cfl(t, cx);
+  int valid = tc_valid(t);
+  if (valid) {
+    tc_update_max_cfl(t, cy);
+    valid = tc_valid(t);
+  }
+  if (rec) {
+    rec->accepted = valid;
+    rec->snapshot = 0;
+    rec->dt_used = dt;
+    rec->cfl = t.CFL;
+  }
+  if (!valid) {
+    tc_update_time(t, 0);
+    tc_reset_cfl(t);
+  } else {
+    tc_update_time(t, 1);
+    if (!t.ended && t.snap_idx <= t.n_snap && t.time >= t.snaps[t.snap_idx - 1]) {
+      if (rec) rec->snapshot = t.snap_idx;
+      t.snap_idx += 1;
+    }
+    tc_reset_cfl(t);
+  }
+  if (rec) {
+    rec->time = t.time;
+    rec->dt_next = t.dt;
+  }
+  return valid;
+}
+
+}  // namespace pcw

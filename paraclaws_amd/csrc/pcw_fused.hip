@@ -1,0 +1,426 @@
+// pcw_fused.hip -- one kernel per time step: xi sweep, eta sweep, source
+// term and physical ghost fill, for any number of instances at once.
+//
+// Mapping (gfx950, wave64, no LDS): a wave owns a strip of 60 output
+// columns plus two halo lanes on each side and marches down the rows of its
+// tile.  Each row is loaded once (coalesced SoA h/hu/hv), swept in xi with
+// DPP lane shifts carrying the neighbour states, and fed into a register
+// window that runs the eta sweep two rows behind.  The eta sweep reads the
+// xi-updated interior and the ghost rows as they were before the step
+// (the reference refreshes ghosts once per step, src/main.f90:246).
+// Results go to the instance's other buffer (ping-pong), so a rejected
+// attempt is rolled back by not flipping the buffer index.
+//
+// Reference stages fused here (src/solver/MOD_solve.f90:23-175 and callees):
+// wave_decomposition/HLL, split_flux, integrate_flux_difference, limiter,
+// second_order_flux_correction, integrator_flux_correction_increment, the
+// CFL max (as max wave speed), integrate_source_term and
+// apply_physical_boundaries.
+#include "pcw_device.hpp"
+#include "pcw_engine.hpp"
+#include "pcw_kernels.hpp"
+
+namespace pcw {
+
+template <typename T>
+__device__ __forceinline__ T ld(const T* p) { return *p; }
+
+template <typename T, bool EXACT, int LIM>
+__global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous
+  // range of tiles so neighbouring strips share that XCD's L2.
+  const int nb = gridDim.x, per = nb >> 3;
+  const int bb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int tid = __builtin_amdgcn_readfirstlane(bb * wpb + (threadIdx.x >> 6));
+  if (tid >= a.ntiles) return;
+  const int4 t = a.tiles[tid];
+  const int in = __builtin_amdgcn_readfirstlane(t.x);
+  const KInst& I = a.inst[in];
+  InstState* S = a.st + in;
+  if (S->status != PCW_STATUS_RUNNING) return;
+
+  const int nx = I.nx, ny = I.ny, pitch = I.pitch;
+  const long long plane = I.plane;
+  const int cur = S->cur;
+  const double dt = S->tc.dt;
+  const T* __restrict__ qin = a.pool + I.off[cur];
+  T* __restrict__ qout = a.pool + I.off[cur ^ 1];
+  const int x0 = t.y, y0 = t.z, y1 = t.w;
+  const int i = x0 - 2 + lane;                 // this lane's column (1-based interior)
+  const int fc = i + 1 < nx + 3 ? i + 1 : nx + 3;  // frame column, clamped for loads
+  const bool out_lane = lane >= 2 && lane <= 61 && i >= 1 && i <= nx;
+  const bool xface = lane >= 1 && i >= 0 && i <= nx + 1;  // interface i-1/2 in the xi CFL range
+  const bool xlim = i >= 1 && i <= nx;         // interface limited (Q7 of the reference)
+  const bool check_cell = i >= -1 && i <= nx + 1;
+
+  const double dtod_x = dt / I.dxi, dtod_y = dt / I.deta;
+  const T dtdx = (T)dtod_x, dtdy = (T)dtod_y;
+  const T eps = (T)a.eps, g = (T)a.g, dry = (T)a.dry;
+  Phys P;
+  P.g = a.g;
+  P.eps = a.eps;
+  P.dry = a.dry;
+  Mesh M;
+  M.xi_min = I.xi_min;
+  M.eta_min = I.eta_min;
+  M.dxi = I.dxi;
+  M.deta = I.deta;
+  const int src = I.source;
+  const bool do_src = (src != PCW_SRC_NONE) || a.wetdry;
+  const T n2 = (T)(I.manning * I.manning);
+  const T* __restrict__ bb_ = (I.boff >= 0) ? a.bpool + I.boff : nullptr;
+  const bool use_b = bb_ && (src == PCW_SRC_BATHY || src == PCW_SRC_BATHY_FRICTION);
+  const int bc_l = I.bc[0], bc_r = I.bc[1], bc_b = I.bc[2], bc_t = I.bc[3];
+  const int l_nx = nx - x0 + 2;                // lane of column nx
+  const bool do_left = x0 == 1;
+  const bool do_right = l_nx >= 3 && l_nx <= 61;
+  const double gam_eta = EXACT ? gamma_eta_exact(M, i) : 1.0;
+
+  T smx = T(0), smy = T(0);
+  int bad = 0;
+
+  // eta-sweep register window
+  T yq[3] = {T(0), T(0), T(0)};      // row r-1 (xi-updated)
+  CellQ<T> yc;                        // eta physics of row r-1
+  yc.s = T(0); yc.f0 = yc.f1 = yc.f2 = T(0); yc.bad = 0;
+  Waves<T> Wa, Wb;                    // interfaces r-1, r-2 (unlimited)
+  Wa.s2 = Wb.s2 = T(0);
+  for (int k = 0; k < 3; ++k) { Wa.w1[k] = Wa.w2[k] = Wb.w1[k] = Wb.w2[k] = T(0); }
+  T Fb[3] = {T(0), T(0), T(0)};       // correction flux at interface r-2
+  T qpp[3] = {T(0), T(0), T(0)};      // row r-2 (xi-updated)
+  double kap_p = 1.0, kap_pp = 1.0;   // kappa(i, r-1), kappa(i, r-2)
+  T fin1[3] = {T(0), T(0), T(0)};     // first finished row (bottom ghosts)
+  T finm[3] = {T(0), T(0), T(0)};     // previous finished row (top ghosts)
+  T bm = T(0), b0 = T(0);             // bathymetry rows r-3, r-2 of this column
+
+  // prefetch first row
+  long long roff = (long long)(y0 - 2 + 1) * pitch + fc;
+  T nh = ld(qin + roff), nhu = ld(qin + plane + roff), nhv = ld(qin + 2 * plane + roff);
+
+  for (int r = y0 - 2; r <= y1 + 2; ++r) {
+    T q[3] = {nh, nhu, nhv};
+    if (r < y1 + 2) {  // prefetch the next row
+      long long o = (long long)(r + 2) * pitch + fc;
+      nh = ld(qin + o);
+      nhu = ld(qin + plane + o);
+      nhv = ld(qin + 2 * plane + o);
+    }
+    T bnew = T(0);
+    if (use_b && r >= y0) bnew = bb_[(long long)r * pitch + fc];  // b(i, r-1)
+
+    const double kap_r = EXACT ? kappa_exact(M, i, r) : 1.0;
+    T qx[3] = {q[0], q[1], q[2]};
+
+    // ------------------------------------------------------ xi sweep
+    if (r >= 1 && r <= ny) {
+      CellQ<T> c;
+      if constexpr (EXACT) c = cell_exact<1>(q[0], q[1], q[2], P);
+      else {
+        if (a.wetdry) c = cell_fast<1, true>(q[0], q[1], q[2], g, dry);
+        else c = cell_fast<1, false>(q[0], q[1], q[2], g, dry);
+      }
+      bad |= (c.bad && check_cell) ? 1 : 0;
+      T qL[3] = {shr1(q[0]), shr1(q[1]), shr1(q[2])};
+      CellQ<T> cL;
+      cL.s = shr1(c.s);
+      cL.f0 = qL[1];
+      cL.f1 = shr1(c.f1);
+      cL.f2 = shr1(c.f2);
+      Waves<T> w;
+      if constexpr (EXACT) w = hll_exact(qL, cL, q, c, P.eps);
+      else w = hll_fast(qL, cL, q, c, eps);
+      smx = (xface && w.s2 > smx) ? w.s2 : smx;
+      T w1R[3], F[3] = {T(0), T(0), T(0)};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) w1R[k] = shl1(w.w1[k]);
+      if (a.ho) {
+        T w2L[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) w2L[k] = shr1(w.w2[k]);
+        T p1 = T(1), p2 = T(1);
+        if (LIM > 0 && xlim) {
+          if constexpr (EXACT) {
+            p1 = limit_exact<LIM>(w.w1, w1R, P.eps);
+            p2 = limit_exact<LIM>(w.w2, w2L, P.eps);
+          } else {
+            p1 = limit_fast<LIM>(w.w1, w1R, eps);
+            p2 = limit_fast<LIM>(w.w2, w2L, eps);
+          }
+        }
+        if constexpr (EXACT) {
+          double ka = shr1(kap_r), kb = kap_r;
+          double al = w.s2;
+          double coef = 0.5 * al * (1.0 - al * dtod_x / (0.5 * (ka + kb)));
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            double t1 = coef * (p1 * w.w1[k]);
+            double t2 = coef * (p2 * w.w2[k]);
+            F[k] = t1 + t2;
+          }
+        } else {
+          T coef = (T(0.5) * w.s2) * fmaf(-w.s2, dtdx, T(1));
+#pragma unroll
+          for (int k = 0; k < 3; ++k) F[k] = coef * fmaf(p1, w.w1[k], p2 * w.w2[k]);
+        }
+      }
+      // all lane shifts happen here, in wave-uniform control flow
+      const T s2R = shl1(w.s2);
+      T FR[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) FR[k] = shl1(F[k]);
+      if (i >= 1 && i <= nx) {
+        if constexpr (EXACT) {
+          const double gam = gamma_xi_exact(M, r);
+          const double fac = dtod_x / kap_r;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            double ap = w.s2 * w.w2[k];
+            double am = (-s2R) * w1R[k];
+            double q1 = q[k] - fac * (gam * ap + gam * am);
+            qx[k] = a.ho ? q1 - dtod_x / kap_r * (FR[k] - F[k]) : q1;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            T d = fmaf(w.s2, w.w2[k], -s2R * w1R[k]) + (FR[k] - F[k]);
+            qx[k] = fmaf(-dtdx, d, q[k]);
+          }
+        }
+      }
+    }
+
+    // ------------------------------------------------------ eta sweep
+    CellQ<T> cy;
+    if constexpr (EXACT) cy = cell_exact<2>(qx[0], qx[1], qx[2], P);
+    else {
+      if (a.wetdry) cy = cell_fast<2, true>(qx[0], qx[1], qx[2], g, dry);
+      else cy = cell_fast<2, false>(qx[0], qx[1], qx[2], g, dry);
+    }
+    // the eta sweep reads rows [0, ny+1] of columns [1, nx]
+    bad |= (cy.bad && out_lane && r <= ny + 1) ? 1 : 0;
+    Waves<T> Wc;
+    Wc.s2 = T(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { Wc.w1[k] = T(0); Wc.w2[k] = T(0); }
+    if (r >= y0 - 1) {
+      if constexpr (EXACT) Wc = hll_exact(yq, yc, qx, cy, P.eps);
+      else Wc = hll_fast(yq, yc, qx, cy, eps);
+      smy = (out_lane && r >= 0 && r <= ny + 1 && Wc.s2 > smy) ? Wc.s2 : smy;
+    }
+    T Fa[3] = {T(0), T(0), T(0)};
+    if (r >= y0 + 1 && a.ho) {  // correction flux at interface r-1
+      T p1 = T(1), p2 = T(1);
+      if (LIM > 0 && r - 1 >= 1 && r - 1 <= ny) {
+        if constexpr (EXACT) {
+          p1 = limit_exact<LIM>(Wa.w1, Wc.w1, P.eps);
+          p2 = limit_exact<LIM>(Wa.w2, Wb.w2, P.eps);
+        } else {
+          p1 = limit_fast<LIM>(Wa.w1, Wc.w1, eps);
+          p2 = limit_fast<LIM>(Wa.w2, Wb.w2, eps);
+        }
+      }
+      if constexpr (EXACT) {
+        double al = Wa.s2;
+        double coef = 0.5 * al * (1.0 - al * dtod_y / (0.5 * (kap_pp + kap_p)));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double t1 = coef * (p1 * Wa.w1[k]);
+          double t2 = coef * (p2 * Wa.w2[k]);
+          Fa[k] = t1 + t2;
+        }
+      } else {
+        T coef = (T(0.5) * Wa.s2) * fmaf(-Wa.s2, dtdy, T(1));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Fa[k] = coef * fmaf(p1, Wa.w1[k], p2 * Wa.w2[k]);
+      }
+    }
+    if (r >= y0 + 2) {  // finish row j = r-2
+      const int j = r - 2;
+      T qn[3];
+      if constexpr (EXACT) {
+        const double fac = dtod_y / kap_pp;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double ap = Wb.s2 * Wb.w2[k];
+          double am = (-Wa.s2) * Wa.w1[k];
+          double q1 = qpp[k] - fac * (gam_eta * ap + gam_eta * am);
+          qn[k] = a.ho ? q1 - dtod_y / kap_pp * (Fa[k] - Fb[k]) : q1;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          T d = fmaf(Wb.s2, Wb.w2[k], -Wa.s2 * Wa.w1[k]) + (Fa[k] - Fb[k]);
+          qn[k] = fmaf(-dtdy, d, qpp[k]);
+        }
+      }
+      if (do_src) {
+        T bxm = shr1(b0), bxp = shl1(b0);
+        apply_source<T>(qn[0], qn[1], qn[2], src, dry, g, (T)I.dxi, (T)I.deta, (T)dt, n2, bxm,
+                        bxp, bm, bnew);
+      }
+      const long long o = (long long)(j + 1) * pitch + (i + 1);
+      if (out_lane) {
+        qout[o] = qn[0];
+        qout[plane + o] = qn[1];
+        qout[2 * plane + o] = qn[2];
+      }
+      // xi-boundary ghosts of row j from this row's finished cells
+      if (do_left || do_right) {
+        T s1v[3], s2v[3], s3v[3];
+        if (do_left) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) { s1v[k] = shl1(qn[k]); s2v[k] = shl1(s1v[k]); s3v[k] = shl1(s2v[k]); }
+          if (lane <= 1) {
+            const int kk = lane == 1 ? 1 : 2;  // ghost layer
+            const int gi = 1 - kk;             // ghost column 0 or -1
+            const double xn = centre(M.xi_min, M.dxi, 1), xf = centre(M.xi_min, M.dxi, 2);
+            const double xg = centre(M.xi_min, M.dxi, gi);
+            const long long og = (long long)(j + 1) * pitch + (gi + 1);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              T nr = lane == 1 ? s1v[k] : s2v[k];
+              T fr = lane == 1 ? s2v[k] : s3v[k];
+              qout[k * plane + og] = ghost_value<T>(bc_l, nr, fr, kk, k == 1, xn, xf, xg);
+            }
+          }
+        }
+        if (do_right) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) { s1v[k] = shr1(qn[k]); s2v[k] = shr1(s1v[k]); s3v[k] = shr1(s2v[k]); }
+          if (lane == l_nx + 1 || lane == l_nx + 2) {
+            const int kk = lane - l_nx;
+            const int gi = nx + kk;
+            const double xn = centre(M.xi_min, M.dxi, nx), xf = centre(M.xi_min, M.dxi, nx - 1);
+            const double xg = centre(M.xi_min, M.dxi, gi);
+            const long long og = (long long)(j + 1) * pitch + (gi + 1);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              T nr = kk == 1 ? s1v[k] : s2v[k];
+              T fr = kk == 1 ? s2v[k] : s3v[k];
+              qout[k * plane + og] = ghost_value<T>(bc_r, nr, fr, kk, k == 1, xn, xf, xg);
+            }
+          }
+        }
+      }
+      // eta-boundary ghosts
+      if (j == 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fin1[k] = qn[k];
+      }
+      if (j == 2 && y0 == 1 && out_lane) {
+        const double yn = centre(M.eta_min, M.deta, 1), yf = centre(M.eta_min, M.deta, 2);
+#pragma unroll
+        for (int kk = 1; kk <= 2; ++kk) {
+          const int gj = 1 - kk;
+          const double yg = centre(M.eta_min, M.deta, gj);
+          const long long og = (long long)(gj + 1) * pitch + (i + 1);
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            qout[k * plane + og] = ghost_value<T>(bc_b, fin1[k], qn[k], kk, k == 2, yn, yf, yg);
+        }
+      }
+      if (j == ny && out_lane) {
+        const double yn = centre(M.eta_min, M.deta, ny), yf = centre(M.eta_min, M.deta, ny - 1);
+#pragma unroll
+        for (int kk = 1; kk <= 2; ++kk) {
+          const int gj = ny + kk;
+          const double yg = centre(M.eta_min, M.deta, gj);
+          const long long og = (long long)(gj + 1) * pitch + (i + 1);
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            qout[k * plane + og] = ghost_value<T>(bc_t, qn[k], finm[k], kk, k == 2, yn, yf, yg);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) finm[k] = qn[k];
+    }
+    // shift the window
+    Wb = Wa;
+    Wa = Wc;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Fb[k] = Fa[k];
+      qpp[k] = yq[k];
+      yq[k] = qx[k];
+    }
+    yc = cy;
+    kap_pp = kap_p;
+    kap_p = kap_r;
+    bm = b0;
+    b0 = bnew;
+  }
+
+  // ------------------------------------------------ reductions + controller
+  const T mx = wave_max(smx), my = wave_max(smy);
+  const int anybad = __any(bad);
+  if (lane == 0) {
+    atomicMax(&S->smax_x, (unsigned long long)__double_as_longlong((double)mx));
+    atomicMax(&S->smax_y, (unsigned long long)__double_as_longlong((double)my));
+    if (anybad) atomicOr(&S->bad, 1);
+  }
+  __threadfence();
+  unsigned int prev = 0;
+  if (lane == 0) prev = atomicAdd(&S->tiles_done, 1u);
+  prev = __shfl(prev, 0, 64);
+  if (prev + 1 != (unsigned)I.ntiles) return;
+  // last tile of this instance: finish the attempt
+  __threadfence();
+  if (lane != 0) return;
+  const double sx = __longlong_as_double((long long)atomicAdd(&S->smax_x, 0ull));
+  const double sy = __longlong_as_double((long long)atomicAdd(&S->smax_y, 0ull));
+  S->smax_x = 0ull;
+  S->smax_y = 0ull;
+  S->last_smax_x = sx;
+  S->last_smax_y = sy;
+  S->tiles_done = 0u;
+  if (atomicAdd(&S->bad, 0) != 0) {
+    S->status = PCW_STATUS_ERROR;
+    S->error = PCW_ERR_STATE;
+    return;
+  }
+  if (!S->controlled) return;
+  pcw_step_record rec;
+  const int ok = tc_attempt(S->tc, sx, sy, I.dxi, I.deta, &rec);
+  S->cfl_last = rec.cfl;
+  if (ok) {
+    S->cur = cur ^ 1;
+    S->accepted += 1;
+  } else {
+    S->rejected += 1;
+  }
+  if (S->log && S->log_n < S->log_cap) S->log[S->log_n++] = rec;
+  if (S->tc.ended) {
+    S->status = PCW_STATUS_ERROR;
+    S->error = PCW_ERR_CONTROL;
+  } else if (!(S->tc.time < S->tc.time_final) ||
+             (S->max_accepted > 0 && S->accepted >= S->max_accepted)) {
+    S->status = PCW_STATUS_DONE;
+  }
+}
+
+// ------------------------------------------------------------- launcher
+template <typename T, bool EXACT>
+static hipError_t launch_lim(int lim, const FusedArgs<T>& a, int blocks, int threads,
+                             hipStream_t s) {
+  switch (lim) {
+    case 0: k_fused_step<T, EXACT, 0><<<blocks, threads, 0, s>>>(a); break;
+    case 1: k_fused_step<T, EXACT, 1><<<blocks, threads, 0, s>>>(a); break;
+    case 2: k_fused_step<T, EXACT, 2><<<blocks, threads, 0, s>>>(a); break;
+    case 3: k_fused_step<T, EXACT, 3><<<blocks, threads, 0, s>>>(a); break;
+    default: k_fused_step<T, EXACT, 4><<<blocks, threads, 0, s>>>(a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fused_f32(int lim, const FusedArgs<float>& a, int blocks, int threads,
+                            hipStream_t s) {
+  return launch_lim<float, false>(lim, a, blocks, threads, s);
+}
+hipError_t launch_fused_f64(int lim, const FusedArgs<double>& a, int blocks, int threads,
+                            hipStream_t s) {
+  return launch_lim<double, true>(lim, a, blocks, threads, s);
+}
+
+}  // namespace pcw

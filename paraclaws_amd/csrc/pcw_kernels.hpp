@@ -1,0 +1,69 @@
+// pcw_kernels.hpp -- launch interfaces between the C ABI and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pcw_engine.hpp"
+
+namespace pcw {
+
+// work-array planes of the staged engine (each one frame of doubles)
+enum {
+  W_LAM = 0,    // lambda(2) = s2 (lambda(1) = -s2)
+  W_W1 = 1,     // 3 planes: wave 1
+  W_W2 = 4,     // 3 planes: wave 2
+  W_AM = 7,     // 3 planes: A-dQ
+  W_AP = 10,    // 3 planes: A+dQ
+  W_T1 = 13,    // 3 planes: limited wave 1
+  W_T2 = 16,    // 3 planes: limited wave 2
+  W_F = 19,     // 3 planes: correction flux
+  W_COUNT = 22
+};
+
+struct StagedArgs {
+  const KInst* inst;   // this instance's descriptor (device)
+  InstState* st;       // this instance's state (device)
+  double* q;           // current buffer, h plane frame origin
+  double* w;           // work planes
+  double g, eps, dry;
+  int limiter;         // 0 = unlimited, 1..4
+  int pad;
+};
+
+template <typename T>
+struct FusedArgs {
+  T* pool;
+  const T* bpool;
+  const KInst* inst;
+  InstState* st;
+  const int4* tiles;   // (instance, x0, y0, y1): output columns x0..x0+59, rows y0..y1
+  int ntiles;
+  int ho;              // add_higher_order_flux_correction
+  int wetdry;          // dry_tol > 0
+  int pad;
+  double g, eps, dry;
+};
+
+hipError_t launch_fused_f32(int lim, const FusedArgs<float>& a, int blocks, int threads,
+                            hipStream_t s);
+hipError_t launch_fused_f64(int lim, const FusedArgs<double>& a, int blocks, int threads,
+                            hipStream_t s);
+hipError_t launch_staged_sweep(const StagedArgs& a, const KInst& hI, int dir, int ho,
+                               hipStream_t s);
+template <typename T>
+hipError_t launch_ghosts(T* pool, const KInst* dinst, InstState* dst, int n, int max_side_cells,
+                         int buf_sel, hipStream_t s);
+template <typename T>
+hipError_t launch_source(T* pool, const T* bpool, const KInst* dinst, InstState* dst, int n,
+                         int max_nx, int max_ny, int wetdry, double g, double dry, hipStream_t s);
+template <typename T>
+hipError_t launch_copy(T* pool, const KInst* dinst, InstState* dst, int n, int to_backup,
+                       int only_if_rejected, hipStream_t s);
+hipError_t launch_st_finish(const KInst* dinst, InstState* dst, int n, hipStream_t s);
+template <typename T>
+hipError_t launch_totals(const T* pool, const KInst* dinst, const InstState* dst, double* part,
+                         int nparts, double* out, hipStream_t s);
+hipError_t launch_set_dt(InstState* dst, const double* ddt, int n, int controlled, hipStream_t s);
+hipError_t launch_flip(InstState* dst, int n, hipStream_t s);
+
+}  // namespace pcw

@@ -1,0 +1,256 @@
+"""GPU parity: the HIP engine through the C ABI against the oracle.
+
+FP64 (fused and staged engines) must reproduce the oracle bit for bit: both
+follow the reference's FP64 operation order with contraction disabled, and
+the reference's CFL-0.9 scheme amplifies last-bit differences (a 1-ulp
+perturbation reaches ~1e-10 relative by t = 0.5), so anything short of
+bit-exactness would not stay within 1e-12.  FP32 is checked against the
+FP64 oracle with the tolerance written at each assertion.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.helpers import rough_state, sides_mask, smooth_state, stable_dt
+
+pytestmark = pytest.mark.gpu
+
+
+def _abi():
+    from paraclaws_amd import abi
+
+    return abi
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _oracle_step(O, cfg, q, dt):
+    """One accepted step of the oracle: solve, source, ghost fill."""
+    ts = O.TimeSettings(dt0=dt, t_final=1e9, n_snap=1, cfl_target=0.9, shrink=1e-9)
+    tc = ts.tc()
+    rc = O.solve(cfg, tc, q)
+    assert rc == 1
+    O.source(cfg, q, None, dt)
+    O.fill_ghosts(cfg, q)
+    return tc.CFL
+
+
+CASES = [
+    # nx, ny, limiter(0 = off), higher_order, bc, state
+    (37, 29, 4, True, (4, 4, 4, 4), "smooth"),
+    (64, 64, 1, True, (2, 2, 2, 2), "rough"),
+    (61, 33, 2, True, (3, 3, 3, 3), "rough"),
+    (130, 71, 3, True, (4, 3, 2, 4), "rough"),
+    (125, 9, 4, True, (2, 4, 3, 2), "smooth"),
+    (8, 200, 4, False, (3, 2, 4, 3), "rough"),
+    (70, 70, 0, True, (4, 4, 4, 4), "rough"),
+]
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}x{c[1]}-lim{c[2]}-ho{int(c[3])}-bc{''.join(map(str, c[4]))}-{c[5]}" for c in CASES])
+def test_fp64_step_bitwise(oracle, engine, case):
+    abi = _abi()
+    O = oracle
+    nx, ny, lim, ho, bc, kind = case
+    q0 = smooth_state(nx, ny, seed=nx + ny) if kind == "smooth" else rough_state(nx, ny, seed=nx * ny)
+    cfg = O.Config(nx=nx, ny=ny, limiter=max(lim, 1), use_limiter=lim > 0, higher_order=ho, bc=bc)
+    O.fill_ghosts(cfg, q0)
+    dt = stable_dt(q0, nx, ny)
+    ref = q0.copy()
+    cfl_ref = _oracle_step(O, cfg, ref, dt)
+    _say(f"fp64 {engine} step {case}")
+    dom = abi.Domain(nx, ny, bc=bc)
+    with abi.Engine([dom], precision=abi.FP64, engine=abi.FUSED if engine == "fused" else abi.STAGED,
+                    higher_order=ho, use_limiter=lim > 0, limiter=max(lim, 1)) as e:
+        e.set_state(0, q0)
+        smax = e.step(dt)[0]
+        e.accept()
+        got = e.get_state(0)
+    m = sides_mask(nx, ny)
+    assert np.array_equal(got[m], ref[m]), np.abs(got[m] - ref[m]).max()
+    dxi, deta = 2.0 / nx, 2.0 / ny
+    cfl = max(smax[0] * dt / dxi, smax[1] * dt / deta)
+    assert cfl == cfl_ref
+
+
+@pytest.mark.parametrize("ixy", [1, 2])
+def test_fp64_single_sweep_bitwise(oracle, ixy):
+    abi = _abi()
+    O = oracle
+    nx, ny = 45, 38
+    q0 = rough_state(nx, ny, seed=7)
+    cfg = O.Config(nx=nx, ny=ny, limiter=4, bc=(3, 4, 2, 3))
+    O.fill_ghosts(cfg, q0)
+    dt = stable_dt(q0, nx, ny)
+    ref = q0.copy()
+    cfl_ref = O.sweep(cfg, ref, ixy, dt)
+    _say(f"fp64 staged single sweep ixy={ixy}")
+    with abi.Engine([abi.Domain(nx, ny, bc=cfg.bc)], precision=abi.FP64, engine=abi.STAGED) as e:
+        e.set_state(0, q0)
+        smax = e.sweep(ixy, dt)[0]
+        got = e.get_state(0)
+    assert np.array_equal(got, ref)
+    assert smax * dt / (2.0 / (nx if ixy == 1 else ny)) == cfl_ref
+
+
+def test_fp32_step_close(oracle):
+    abi = _abi()
+    O = oracle
+    nx, ny = 150, 90
+    q0 = smooth_state(nx, ny, seed=3)
+    cfg = O.Config(nx=nx, ny=ny, limiter=4, bc=(4, 4, 4, 4))
+    O.fill_ghosts(cfg, q0)
+    dt = stable_dt(q0, nx, ny)
+    ref = q0.copy()
+    _oracle_step(O, cfg, ref, dt)
+    _say("fp32 fused step")
+    with abi.Engine([abi.Domain(nx, ny, bc=cfg.bc)], precision=abi.FP32, limiter=4) as e:
+        e.set_state(0, q0)
+        e.step(dt)
+        e.accept()
+        got = e.get_state(0)
+    inner = (slice(2, -2), slice(2, -2))
+    for k in range(3):
+        d = np.abs(got[inner][..., k] - ref[inner][..., k])
+        # FP32 state: per-cell error within a few ulp of the field scale
+        assert d.max() <= 4e-6 * max(1.0, np.abs(ref[inner][..., k]).max()), (k, d.max())
+
+
+def _config1_like(O, n, t_final):
+    cfg = O.Config(nx=n, ny=n, limiter=4, bc=(4, 4, 4, 4))
+    ts = O.TimeSettings(t_final=t_final, cfl_target=0.9, dt_max=1e-2, n_snap=10)
+    return cfg, ts
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_fp64_device_controlled_trajectory(oracle, engine):
+    """Config-1 physics (dam break, MC, walls, CFL 0.9) at 64^2 to t = 0.1:
+    identical dt/CFL/time sequence and final state."""
+    abi = _abi()
+    O = oracle
+    n = 64
+    cfg, ts = _config1_like(O, n, 0.1)
+    q = O.dam_break(cfg)
+    q_init = q.copy()
+    recs, tc = O.run(cfg, ts, q)
+    _say(f"fp64 {engine} device-controlled run 64^2")
+    with abi.Engine([abi.Domain(n, n, bc=cfg.bc)], precision=abi.FP64,
+                    engine=abi.FUSED if engine == "fused" else abi.STAGED, limiter=4) as e:
+        e.set_state(0, q_init)
+        e.fill_ghosts()
+        e.time_init(t_final=ts.t_final, dt0=ts.dt0, cfl_target=ts.cfl_target, dt_max=ts.dt_max,
+                    dt_min=ts.dt_min, growth=ts.growth, shrink=ts.shrink, n_snapshots=ts.n_snap)
+        e.enable_log(len(recs) + 16)
+        e.run(steps_per_graph=16)
+        log = e.read_log(0)
+        st = e.status(0)
+        got = e.get_state(0)
+    assert st.status == abi.STATUS_DONE
+    assert len(log) == len(recs)
+    assert log == recs
+    assert np.array_equal(got[sides_mask(n, n)], q[sides_mask(n, n)])
+
+
+def test_fp64_host_controlled_trajectory_with_rejection(oracle):
+    """Host-controlled loop (pcw_step/accept/reject) with a first step forced
+    to be rejected (dt0 far above the CFL limit)."""
+    abi = _abi()
+    from paraclaws_amd.timectl import TimeControl, run_host_controlled
+
+    O = oracle
+    n = 48
+    cfg = O.Config(nx=n, ny=n, limiter=2, bc=(2, 2, 2, 2))
+    ts = O.TimeSettings(t_final=0.05, dt0=0.05, cfl_target=0.9, dt_max=1e-2, n_snap=5)
+    q = O.dam_break(cfg)
+    q_init = q.copy()
+    recs, _ = O.run(cfg, ts, q)
+    assert recs[0][0] == 0, "the first attempt must be rejected"
+    _say("fp64 fused host-controlled run with rejection")
+    tc = TimeControl(0.0, ts.t_final, ts.dt0, ts.n_snap, ts.cfl_target, ts.dt_max, ts.dt_min,
+                     ts.growth, ts.shrink, True)
+    with abi.Engine([abi.Domain(n, n, bc=cfg.bc)], precision=abi.FP64, limiter=2) as e:
+        e.set_state(0, q_init)
+        e.fill_ghosts()
+        got_recs = run_host_controlled(e, tc, 2.0 / n, 2.0 / n)
+        got = e.get_state(0)
+    assert got_recs == recs
+    assert np.array_equal(got[sides_mask(n, n)], q[sides_mask(n, n)])
+
+
+def test_batch_equals_solo_and_oracle(oracle):
+    """Instances of uneven sizes in one handle: each equals the same instance
+    run alone (bit for bit, FP32 and FP64), and the FP64 batch equals the oracle."""
+    abi = _abi()
+    O = oracle
+    sizes = [(40, 40, 1.8), (97, 61, 3.0), (64, 130, 2.5), (23, 18, 6.0)]
+    doms, qs, cfgs = [], [], []
+    for nx, ny, r in sizes:
+        cfg = O.Config(nx=nx, ny=ny, limiter=4, bc=(4, 4, 4, 4))
+        q = O.dam_break(cfg, h_in=r)
+        O.fill_ghosts(cfg, q)
+        doms.append(abi.Domain(nx, ny, bc=cfg.bc))
+        qs.append(q)
+        cfgs.append(cfg)
+    tkw = dict(t_final=0.03, dt0=1e-4, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10, growth=1.2,
+               shrink=0.5, n_snapshots=3)
+    for prec in (abi.FP64, abi.FP32):
+        _say(f"batch precision={prec}")
+        with abi.Engine(doms, precision=prec, limiter=4) as e:
+            for k, q in enumerate(qs):
+                e.set_state(k, q)
+            e.time_init(**tkw)
+            e.run(steps_per_graph=8)
+            batch = [e.get_state(k) for k in range(len(doms))]
+            stats = [e.status(k) for k in range(len(doms))]
+        for k, d in enumerate(doms):
+            assert stats[k].status == abi.STATUS_DONE
+            with abi.Engine([d], precision=prec, limiter=4) as e1:
+                e1.set_state(0, qs[k])
+                e1.time_init(**tkw)
+                e1.run(steps_per_graph=8)
+                solo = e1.get_state(0)
+            assert np.array_equal(batch[k], solo), (prec, k)
+            if prec == abi.FP64:
+                ref = qs[k].copy()
+                ts = O.TimeSettings(t_final=tkw["t_final"], dt0=tkw["dt0"], n_snap=3)
+                O.run(cfgs[k], ts, ref)
+                m = sides_mask(d.nx, d.ny)
+                assert np.array_equal(batch[k][m], ref[m]), k
+
+
+def test_totals(oracle):
+    abi = _abi()
+    nx, ny = 300, 170
+    q = smooth_state(nx, ny, seed=11)
+    for prec in (abi.FP64, abi.FP32):
+        with abi.Engine([abi.Domain(nx, ny)], precision=prec) as e:
+            e.set_state(0, q)
+            tot = e.totals(0)
+        qq = q if prec == abi.FP64 else q.astype(np.float32).astype(np.float64)
+        ref = qq[2:-2, 2:-2].reshape(-1, 3).sum(axis=0)
+        assert np.allclose(tot, ref, rtol=1e-13, atol=1e-12), (prec, tot, ref)
+
+
+def test_error_on_dry_cell(oracle):
+    """h = 0 in the interior: the reference stops ("isnan in jakobi");
+    the FP64 engine reports PCW_ERR_STATE."""
+    abi = _abi()
+    nx = ny = 20
+    q = smooth_state(nx, ny, seed=1)
+    q[10, 10, 0] = 0.0
+    with abi.Engine([abi.Domain(nx, ny)], precision=abi.FP64) as e:
+        e.set_state(0, q)
+        with pytest.raises(abi.PcwError) as ei:
+            e.step(1e-4)
+        assert ei.value.code == abi.PCW_ERR_STATE
+
+
+def test_user_bc_rejected():
+    abi = _abi()
+    with pytest.raises(abi.PcwError) as ei:
+        abi.Engine([abi.Domain(16, 16, bc=(1, 2, 2, 2))])
+    assert ei.value.code == abi.PCW_ERR_BC
