@@ -99,6 +99,10 @@ def lib():
         L.orc_tc_reset_cfl.restype = None
         L.orc_run.argtypes = [P(OrcCfg), P(OrcTc), dp, dp, C.c_int32, C.c_int32,
                               P(OrcStepRec), C.c_int32]
+        L.orc_cell_phys.argtypes = [P(OrcCfg), C.c_int, dp, dp, dp]
+        L.orc_hll.argtypes = [P(OrcCfg), C.c_int, dp, dp, dp]
+        L.orc_phi.argtypes = [C.c_int, C.c_double]
+        L.orc_phi.restype = C.c_double
         L.orc_totals.argtypes = [P(OrcCfg), dp, dp]
         L.orc_totals.restype = None
         _lib = L
@@ -234,3 +238,27 @@ def totals(cfg: Config, q: np.ndarray):
     c = cfg.c()
     lib().orc_totals(C.byref(c), _dp(q), _dp(out))
     return out
+
+
+def cell_phys(cfg: Config, ixy: int, q):
+    """(rc, speed, flux) of one cell for sweep ixy."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    s = C.c_double()
+    f = np.zeros(3)
+    c = cfg.c()
+    rc = lib().orc_cell_phys(C.byref(c), ixy, _dp(q), C.byref(s), _dp(f))
+    return rc, s.value, f
+
+
+def hll(cfg: Config, ixy: int, qL, qR):
+    """(rc, lambda[2], W1[3], W2[3]) of one interface."""
+    qL = np.ascontiguousarray(qL, dtype=np.float64)
+    qR = np.ascontiguousarray(qR, dtype=np.float64)
+    out = np.zeros(8)
+    c = cfg.c()
+    rc = lib().orc_hll(C.byref(c), ixy, _dp(qL), _dp(qR), _dp(out))
+    return rc, out[0:2], out[2:5], out[5:8]
+
+
+def phi(code: int, r: float) -> float:
+    return lib().orc_phi(code, r)

@@ -782,3 +782,22 @@ void orc_totals(const orc_cfg* c, const double* vars, double out[3]) {
   out[1] = s1;
   out[2] = s2;
 }
+
+/* ------------------------------------------------------------ test hooks */
+int orc_cell_phys(const orc_cfg* c, int ixy, const double q[3], double* s, double f[3]) {
+  return (ixy == 1) ? phys_x(c, q, s, f) : phys_y(c, q, s, f);
+}
+
+int orc_hll(const orc_cfg* c, int ixy, const double qL[3], const double qR[3], double out[8]) {
+  double lam[2], wav[6];
+  work_t w;
+  w.lambda = lam;
+  w.waves = wav;
+  int rc = hll(c, ixy, qL, qR, &w, 0);
+  out[0] = lam[0];
+  out[1] = lam[1];
+  for (int k = 0; k < 6; ++k) out[2 + k] = wav[k];
+  return rc;
+}
+
+double orc_phi(int code, double r) { return phi_of(code, r); }

@@ -117,6 +117,13 @@ int orc_run(const orc_cfg* c, orc_tc* tc, double* vars, const double* bathy,
             int32_t max_accepted, int32_t max_attempts, orc_step_rec* rec,
             int32_t rec_cap);
 
+/* Test hooks: per-cell speed and flux of one sweep direction (returns the
+ * reference's error condition), one HLL interface (out: lambda1, lambda2,
+ * W1[3], W2[3]) and the limiter function phi(code, r). */
+int orc_cell_phys(const orc_cfg* c, int ixy, const double q[3], double* s, double f[3]);
+int orc_hll(const orc_cfg* c, int ixy, const double qL[3], const double qR[3], double out[8]);
+double orc_phi(int code, double r);
+
 /* FP64 totals of the interior: sum h, sum hu, sum hv (cell-area weighted
  * by kappa*dxi*deta). */
 void orc_totals(const orc_cfg* c, const double* vars, double out[3]);

@@ -30,10 +30,12 @@ def smooth_state(nx, ny, seed=0, amp=0.3, vel=0.3):
 
 
 def rough_state(nx, ny, seed=0):
-    """Piecewise state with jumps in every field (exercises every limiter branch)."""
+    """Piecewise state with jumps in every field (exercises every limiter
+    branch).  Depths stay in [1, 1.25] so that linear extrapolation into the
+    ghost cells (BC 3: 3 q1 - 2 q2) keeps them positive."""
     rng = np.random.default_rng(seed)
     q = np.zeros((ny + 4, nx + 4, 3))
-    q[..., 0] = rng.choice([1.0, 1.4, 2.0, 3.0], size=(ny + 4, nx + 4)) + 0.05 * rng.random((ny + 4, nx + 4))
+    q[..., 0] = rng.choice([1.0, 1.1, 1.2], size=(ny + 4, nx + 4)) + 0.05 * rng.random((ny + 4, nx + 4))
     q[..., 1] = rng.normal(0.0, 0.4, size=(ny + 4, nx + 4))
     q[..., 2] = rng.normal(0.0, 0.4, size=(ny + 4, nx + 4))
     return q

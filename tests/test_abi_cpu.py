@@ -1,0 +1,78 @@
+"""CPU checks of the C ABI: the HIP library loads without a GPU, exports every
+entry point include/pcw.h declares, and the ctypes mirrors of the structs
+have the C layout (checked with a gcc-compiled probe of the header)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pcw.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(pcw_\w+)\s*\(", text, re.M)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    from paraclaws_amd import abi
+
+    L = abi.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(abi.SIGNATURES), set(names) ^ set(abi.SIGNATURES)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    from paraclaws_amd import abi
+
+    probe = tmp_path / "probe.c"
+    probe.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "pcw.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu\\n\","
+        "sizeof(pcw_domain), sizeof(pcw_params), sizeof(pcw_time), sizeof(pcw_step_record),"
+        "sizeof(pcw_status), offsetof(pcw_params, grav), offsetof(pcw_domain, xi_min),"
+        "offsetof(pcw_time, n_snapshots));return 0;}\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(probe), "-o", str(exe)], check=True)
+    got = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()))
+    want = [C.sizeof(abi.PcwDomain), C.sizeof(abi.PcwParams), C.sizeof(abi.PcwTime),
+            C.sizeof(abi.PcwStepRecord), C.sizeof(abi.PcwStatus), abi.PcwParams.grav.offset,
+            abi.PcwDomain.xi_min.offset, abi.PcwTime.n_snapshots.offset]
+    assert got == want
+
+
+def test_create_rejects_bad_arguments_before_touching_the_gpu():
+    from paraclaws_amd import abi
+
+    with pytest.raises(abi.PcwError) as e:
+        abi.Engine([abi.Domain(16, 16, bc=(1, 2, 2, 2))])
+    assert e.value.code == abi.PCW_ERR_BC
+    with pytest.raises(abi.PcwError) as e:
+        abi.Engine([abi.Domain(16, 16)], precision=abi.FP32, engine=abi.STAGED)
+    assert e.value.code == abi.PCW_ERR_ARG
+    with pytest.raises(abi.PcwError) as e:
+        abi.Engine([abi.Domain(1, 16)])
+    assert e.value.code == abi.PCW_ERR_ARG
+    with pytest.raises(abi.PcwError) as e:
+        abi.Engine([abi.Domain(16, 16)], limiter=7)
+    assert e.value.code == abi.PCW_ERR_ARG
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    from paraclaws_amd import abi
+
+    saved = abi._lib
+    abi._lib = None
+    try:
+        with pytest.raises(RuntimeError, match="no CPU fallback"):
+            abi.load(str(tmp_path / "nope.so"))
+    finally:
+        abi._lib = saved

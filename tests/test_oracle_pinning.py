@@ -1,0 +1,414 @@
+"""Pins the oracle before it is trusted as the checker.
+
+No reference-held golden vectors exist for this path (the reference ships no
+tests or fixtures) and the reference cannot be built here (it needs the
+un-vendored mpidcl library and an mpi_f08 module).  The oracle is pinned by:
+  * LAPACK dgeev / BLAS dgemv on the reference's Jacobian matrices (a3-a5),
+  * closed forms: the epsilon-regularised HLL on uniform states (Q3), the
+    wave-sum and flux-difference identities, the textbook limiter values with
+    the reference's code numbering (Q6), the BC formulas, cell centres,
+  * invariants: mirror and transpose symmetry, the mass budget of a sweep,
+  * an independent reference-structured restatement (tests/refstruct.py:
+    per-interface Jacobian matrices, dgeev, dgemv, the serial limiter with
+    its wnorm2r carry) run on tiny grids,
+  * table-driven known-answer tests of the time controller (exact binary
+    fractions, so every expected value is written out).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from scipy.linalg import blas, lapack
+
+from tests import refstruct
+from tests.helpers import GRAV, rough_state, smooth_state, stable_dt
+
+
+def jac(ixy, q, g=GRAV):
+    return refstruct.get_jakobi(ixy, q, g)
+
+
+# ------------------------------------------------------------- a3 / a4 / a5
+@pytest.mark.parametrize("ixy", [1, 2])
+def test_speed_and_flux_vs_lapack_blas(oracle, ixy):
+    rng = np.random.default_rng(ixy)
+    cfg = oracle.Config()
+    for _ in range(300):
+        q = np.array([rng.uniform(0.05, 10.0), rng.normal(0, 3), rng.normal(0, 3)])
+        rc, s, f = oracle.cell_phys(cfg, ixy, q)
+        assert rc == 0
+        J = jac(ixy, q)
+        wr, wi, _, _, info = lapack.dgeev(np.asfortranarray(J), compute_vl=0, compute_vr=1)
+        assert info == 0 and np.all(np.abs(wi) <= 1e-10)
+        # dgeev is backward stable: |error| <= c * eps * ||J||
+        assert s == pytest.approx(np.abs(wr).max(), rel=0, abs=8 * 2.2e-16 * np.abs(J).sum())
+        fb = blas.dgemv(1.0, np.asfortranarray(J), q)
+        assert np.allclose(f, fb, rtol=1e-15, atol=1e-14)
+
+
+def test_flux_is_J_times_q_not_physical_flux(oracle):
+    """Q1: the reference's flux J*q carries g h^2 (not g h^2 / 2)."""
+    cfg = oracle.Config()
+    h, hu, hv = 2.0, 0.5, -0.25
+    _, _, fx = oracle.cell_phys(cfg, 1, [h, hu, hv])
+    _, _, fy = oracle.cell_phys(cfg, 2, [h, hu, hv])
+    g = GRAV
+    assert fx == pytest.approx([hu, hu * hu / h + g * h * h, hu * hv / h], rel=1e-15)
+    assert fy == pytest.approx([hv, hu * hv / h, hv * hv / h + g * h * h], rel=1e-15)
+    assert g == 9.8100004196166992  # REAL*4 9.81 widened (Q5)
+
+
+def test_nan_and_complex_eigen_conditions(oracle):
+    cfg = oracle.Config()
+    assert oracle.cell_phys(cfg, 1, [0.0, 0.0, 0.0])[0] != 0  # isnan in jakobi
+    assert oracle.cell_phys(cfg, 1, [-1.0, 0.1, 0.0])[0] != 0  # complex eigenvalues
+    # tiny negative depth: imaginary part below ud_precision, not an error
+    rc, s, _ = oracle.cell_phys(cfg, 1, [-1e-25, 1e-27, 0.0])
+    assert rc == 0 and s == pytest.approx(abs(1e-27 / -1e-25))
+
+
+# ------------------------------------------------------------------- a2 HLL
+@pytest.mark.parametrize("ixy", [1, 2])
+def test_hll_uniform_state_epsilon_leak(oracle, ixy):
+    """Q3: on a uniform state W1 = q eps/(2 s2 - eps) = -W2."""
+    cfg = oracle.Config()
+    q = np.array([1.7, 0.4, -0.3])
+    rc, lam, w1, w2 = oracle.hll(cfg, ixy, q, q)
+    s2 = lam[1]
+    assert lam[0] == -s2
+    eps = 1e-10
+    expect = q * eps / (2 * s2 - eps)
+    # W1 = q* - q cancels ~11 digits of q* (|W1|/|q| ~ 1e-11)
+    assert np.allclose(w1, expect, rtol=1e-4, atol=0)
+    assert np.allclose(w2, -expect, rtol=1e-4, atol=0)
+
+
+@pytest.mark.parametrize("ixy", [1, 2])
+def test_hll_identities(oracle, ixy):
+    """W1 + W2 = qR - qL and s1 W1 + s2 W2 = fR - fL - eps q*."""
+    rng = np.random.default_rng(10 + ixy)
+    cfg = oracle.Config()
+    for _ in range(200):
+        qL = np.array([rng.uniform(0.2, 5), rng.normal(0, 2), rng.normal(0, 2)])
+        qR = np.array([rng.uniform(0.2, 5), rng.normal(0, 2), rng.normal(0, 2)])
+        _, lam, w1, w2 = oracle.hll(cfg, ixy, qL, qR)
+        _, sL, fL = oracle.cell_phys(cfg, ixy, qL)
+        _, sR, fR = oracle.cell_phys(cfg, ixy, qR)
+        assert lam[1] == max(sL, sR) and lam[0] == -lam[1]
+        scale = np.abs(qR - qL).max() + 1.0
+        assert np.allclose(w1 + w2, qR - qL, atol=1e-13 * scale, rtol=0)
+        qs = qL + w1
+        fscale = np.abs(fR - fL).max() + lam[1] * scale
+        assert np.allclose(lam[0] * w1 + lam[1] * w2, fR - fL - 1e-10 * qs, atol=1e-13 * fscale, rtol=0)
+
+
+def test_hll_zero_speed_skip(oracle):
+    """Q4: |s| <= ud_precision gives lambda = 0, W = 0."""
+    cfg = oracle.Config(grav=0.0)
+    rc, lam, w1, w2 = oracle.hll(cfg, 1, [1.0, 0.0, 0.0], [2.0, 0.0, 0.0])
+    assert rc == 0 and np.all(lam == 0) and np.all(w1 == 0) and np.all(w2 == 0)
+
+
+# --------------------------------------------------------------- a8 limiter
+LIMITER_TABLE = {
+    # code: phi(-1), phi(0), phi(0.5), phi(1), phi(2), phi(10)
+    1: [0.0, 0.0, 0.5, 1.0, 1.0, 1.0],                  # minmod
+    2: [0.0, 0.0, 1.0, 1.0, 2.0, 2.0],                  # superbee (code 2 in the source, Q6)
+    3: [0.0, 0.0, 1.0 / 1.5, 1.0, 4.0 / 3.0, 20.0 / 11.0],  # van Leer (code 3)
+    4: [0.0, 0.0, 0.75, 1.0, 1.5, 2.0],                 # MC
+}
+
+
+@pytest.mark.parametrize("code", [1, 2, 3, 4])
+def test_limiter_textbook_values(oracle, code):
+    rs = [-1.0, 0.0, 0.5, 1.0, 2.0, 10.0]
+    got = [oracle.phi(code, r) for r in rs]
+    assert got == pytest.approx(LIMITER_TABLE[code], rel=1e-15)
+
+
+# --------------------------------------------------------- BCs and mesh
+def test_ghost_fill_closed_forms(oracle):
+    nx, ny = 9, 7
+    rng = np.random.default_rng(5)
+    q0 = rng.normal(1.0, 0.2, size=(ny + 4, nx + 4, 3))
+    for codes in [(2, 2, 2, 2), (3, 3, 3, 3), (4, 4, 4, 4)]:
+        cfg = oracle.Config(nx=nx, ny=ny, bc=codes)
+        q = q0.copy()
+        oracle.fill_ghosts(cfg, q)
+        c = codes[0]
+        I = slice(2, -2)
+        left = [q[I, 1], q[I, 0]]    # ghost columns 0, -1
+        right = [q[I, -2], q[I, -1]]  # nx+1, nx+2
+        bot = [q[1, I], q[0, I]]
+        top = [q[-2, I], q[-1, I]]
+        c1, c2 = q0[I, 2], q0[I, 3]
+        d1, d2 = q0[I, -3], q0[I, -4]
+        r1, r2 = q0[2, I], q0[3, I]
+        t1, t2 = q0[-3, I], q0[-4, I]
+        if c == 2:
+            for a, near in ((left, c1), (right, d1), (bot, r1), (top, t1)):
+                assert np.array_equal(a[0], near) and np.array_equal(a[1], near)
+        elif c == 3:  # linear extrapolation on a uniform mesh
+            for a, n1, n2 in ((left, c1, c2), (right, d1, d2), (bot, r1, r2), (top, t1, t2)):
+                assert np.allclose(a[0], 2 * n1 - n2, atol=1e-13)
+                assert np.allclose(a[1], 3 * n1 - 2 * n2, atol=1e-13)
+        else:  # wall: mirror, normal momentum negated
+            sgn_x = np.array([1, -1, 1])
+            sgn_y = np.array([1, 1, -1])
+            assert np.array_equal(left[0], c1 * sgn_x) and np.array_equal(left[1], c2 * sgn_x)
+            assert np.array_equal(right[0], d1 * sgn_x) and np.array_equal(right[1], d2 * sgn_x)
+            assert np.array_equal(bot[0], r1 * sgn_y) and np.array_equal(bot[1], r2 * sgn_y)
+            assert np.array_equal(top[0], t1 * sgn_y) and np.array_equal(top[1], t2 * sgn_y)
+        for a in (slice(0, 2), slice(-2, None)):  # corners are never filled
+            for b in (slice(0, 2), slice(-2, None)):
+                assert np.array_equal(q[a, b], q0[a, b])
+
+
+def test_user_bc_aborts(oracle):
+    cfg = oracle.Config(nx=4, ny=4, bc=(1, 2, 2, 2))
+    with pytest.raises(RuntimeError):
+        oracle.fill_ghosts(cfg, cfg.frame())
+
+
+def test_mesh_metrics_and_centres(oracle):
+    # power-of-two spacing: every metric is exactly one
+    m = oracle.metrics(oracle.Config(nx=256, ny=256))
+    assert np.all(m["kappa"] == 1.0) and np.all(m["gamma_xi"] == 1.0) and np.all(m["gamma_eta"] == 1.0)
+    assert m["xi"][0, 2] == -1.0 + 1.0 / 256 and m["xi"][0, -3] == 1.0 - 1.0 / 256
+    # spacing 2/100 is inexact: metrics within a few ulp of one
+    m = oracle.metrics(oracle.Config(nx=100, ny=70))
+    for k in ("kappa", "gamma_xi", "gamma_eta"):
+        assert np.abs(m[k] - 1.0).max() < 1e-12
+
+
+def test_dam_break_mask(oracle):
+    cfg = oracle.Config(nx=256, ny=256)
+    q = oracle.dam_break(cfg)
+    x = -1.0 + (np.arange(-1, 259) - 1) * (2 / 256) + (2 / 256) / 2
+    inside = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2) < 0.5
+    assert np.array_equal(q[..., 0] == 2.0, inside)
+    assert np.all(q[..., 1:] == 0.0)
+
+
+# --------------------------------------------------------- invariants
+def test_uniform_state_sweep_leak(oracle):
+    """A uniform state with zero-gradient BCs gains exactly
+    q * (dt/dxi) * 2 s2 eps / (2 s2 - eps) per sweep (the eps leak, Q3)."""
+    nx = ny = 12
+    cfg = oracle.Config(nx=nx, ny=ny, bc=(2, 2, 2, 2))
+    q = cfg.frame()
+    q[...] = [1.3, 0.0, 0.0]
+    dt = 1e-3
+    oracle.sweep(cfg, q, 1, dt)
+    s2 = np.sqrt(GRAV * 1.3)
+    gain = 1.3 * (dt / (2 / nx)) * 2 * s2 * 1e-10 / (2 * s2 - 1e-10)
+    assert np.allclose(q[2:-2, 2:-2, 0] - 1.3, gain, rtol=1e-4, atol=0)
+    assert np.all(q[2:-2, 2:-2, 0] > 1.3)
+
+
+def _mirror_solve(oracle, q, nx, ny, lim, use_lim, ho=True):
+    cfg = oracle.Config(nx=nx, ny=ny, limiter=lim, use_limiter=use_lim, higher_order=ho,
+                        bc=(4, 3, 2, 2))
+    cfgm = oracle.Config(nx=nx, ny=ny, limiter=lim, use_limiter=use_lim, higher_order=ho,
+                         bc=(3, 4, 2, 2))
+    q = q.copy()
+    oracle.fill_ghosts(cfg, q)
+    qm = q[:, ::-1].copy()
+    qm[..., 1] *= -1
+    dt = stable_dt(q, nx, ny)
+    tc = oracle.TimeSettings(dt0=dt, shrink=1e-9).tc()
+    tcm = oracle.TimeSettings(dt0=dt, shrink=1e-9).tc()
+    assert oracle.solve(cfg, tc, q) == 1 and oracle.solve(cfgm, tcm, qm) == 1
+    back = qm[:, ::-1].copy()
+    back[..., 1] *= -1
+    # no CFL comparison: the xi range (Q10) holds the ghost-ghost face -1|0 on
+    # the left but not nx+1|nx+2 on the right, so the CFL is not symmetric
+    return q[2:-2, 2:-2], back[2:-2, 2:-2]
+
+
+@pytest.mark.parametrize("ho,use_lim", [(False, False), (True, False)])
+def test_mirror_symmetry_without_limiter(oracle, ho, use_lim):
+    nx, ny = 14, 11
+    a, b = _mirror_solve(oracle, rough_state(nx, ny, seed=2), nx, ny, 4, use_lim, ho)
+    assert np.allclose(a, b, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("lim", [1, 2, 3, 4])
+def test_mirror_asymmetry_is_the_unlimited_right_face(oracle, lim):
+    """Q7: faces 1..nx are limited, face nx+1 is not, so mirroring x changes
+    only the two edge columns; every other column is mirror symmetric."""
+    nx, ny = 14, 11
+    a, b = _mirror_solve(oracle, rough_state(nx, ny, seed=lim), nx, ny, lim, True)
+    assert np.allclose(a[:, 1:-1], b[:, 1:-1], rtol=1e-13, atol=1e-13)
+    assert not np.allclose(a[:, [0, -1]], b[:, [0, -1]], rtol=1e-13, atol=1e-13)
+
+
+def test_transpose_symmetry_single_sweep(oracle):
+    nx, ny = 13, 17
+    q = rough_state(nx, ny, seed=3)
+    cfg = oracle.Config(nx=nx, ny=ny, limiter=4, bc=(3, 4, 2, 4))
+    cfgt = oracle.Config(nx=ny, ny=nx, limiter=4, bc=(2, 4, 3, 4))
+    oracle.fill_ghosts(cfg, q)
+    qt = np.ascontiguousarray(q.transpose(1, 0, 2)[..., [0, 2, 1]])
+    dt = stable_dt(q, nx, ny)
+    c1 = oracle.sweep(cfg, q, 1, dt)
+    c2 = oracle.sweep(cfgt, qt, 2, dt * (ny / nx) * (nx / ny))
+    back = np.ascontiguousarray(qt.transpose(1, 0, 2)[..., [0, 2, 1]])
+    I = (slice(2, -2), slice(2, -2))
+    assert np.allclose(back[I], q[I], rtol=1e-13, atol=1e-13)
+
+
+def test_sweep_mass_budget(oracle):
+    """First-order xi sweep: per row, the change of sum(h) equals
+    -dt/dxi * (A+dQ at the first face + A-dQ at the last face
+    + sum over inner faces of (f_R - f_L - eps q*)), from per-interface waves."""
+    nx, ny = 16, 5
+    cfg = oracle.Config(nx=nx, ny=ny, higher_order=False, bc=(3, 2, 2, 2))
+    q = rough_state(nx, ny, seed=9)
+    oracle.fill_ghosts(cfg, q)
+    dt = stable_dt(q, nx, ny)
+    q1 = q.copy()
+    oracle.sweep(cfg, q1, 1, dt)
+    dxi = 2.0 / nx
+    for j in range(1, ny + 1):
+        row = q[j + 1]
+        total = 0.0
+        for i in range(1, nx + 2):  # faces i-1/2, i = 1..nx+1
+            _, lam, w1, w2 = oracle.hll(cfg, 1, row[i], row[i + 1])
+            if i == 1:
+                total += lam[1] * w2[0]          # A+dQ of the first face
+            elif i == nx + 1:
+                total += lam[0] * w1[0]          # A-dQ of the last face
+            else:
+                _, _, fL = oracle.cell_phys(cfg, 1, row[i])
+                _, _, fR = oracle.cell_phys(cfg, 1, row[i + 1])
+                total += fR[0] - fL[0] - 1e-10 * (row[i][0] + w1[0])
+        dmass = q1[j + 1, 2:-2, 0].sum() - row[2:-2, 0].sum()
+        assert dmass == pytest.approx(-dt / dxi * total, rel=1e-11, abs=1e-13)
+
+
+# ------------------------------------- independent reference-structured check
+REFSTRUCT_CASES = [
+    (10, 8, 4, True, True, (2, 2, 2, 2)),
+    (9, 7, 1, True, True, (3, 4, 2, 3)),
+    (8, 9, 2, True, True, (4, 4, 4, 4)),
+    (7, 10, 3, True, True, (2, 3, 4, 2)),
+    (8, 8, 4, True, False, (2, 2, 3, 3)),
+    (9, 6, 4, False, True, (4, 2, 2, 4)),
+]
+
+
+@pytest.mark.parametrize("case", REFSTRUCT_CASES)
+def test_oracle_matches_reference_structured(oracle, case):
+    nx, ny, lim, ho, use_lim, bc = case
+    cfg = oracle.Config(nx=nx, ny=ny, limiter=lim, higher_order=ho, use_limiter=use_lim, bc=bc)
+    for seed in range(2):
+        q = rough_state(nx, ny, seed=100 + seed) if seed else smooth_state(nx, ny, seed=seed)
+        oracle.fill_ghosts(cfg, q)
+        dt = stable_dt(q, nx, ny)
+        a, b = q.copy(), q.copy()
+        tc = oracle.TimeSettings(dt0=dt, shrink=0.5, cfl_target=0.9).tc()
+        rc_a = oracle.solve(cfg, tc, a)
+        rc_b, cfl_b = refstruct.solve(b, dt, nx, ny, GRAV, ho=ho, use_lim=use_lim, lim=lim)
+        assert rc_a == rc_b
+        assert tc.CFL == pytest.approx(cfl_b, rel=1e-14)
+        assert np.allclose(a, b, rtol=1e-13, atol=1e-13), np.abs(a - b).max()
+
+
+def test_limiter_range_and_unlimited_edges(oracle):
+    """Q7: with an absurd limiter (phi would be 0), only faces 1..nx are
+    limited; comparing HO-with-limiter against the refstruct also covers the
+    edge faces, so here check limiting actually changes the result."""
+    nx, ny = 10, 6
+    q = rough_state(nx, ny, seed=1)
+    a = q.copy()
+    b = q.copy()
+    c1 = oracle.Config(nx=nx, ny=ny, limiter=1, use_limiter=True)
+    c2 = oracle.Config(nx=nx, ny=ny, limiter=1, use_limiter=False)
+    oracle.fill_ghosts(c1, a)
+    oracle.fill_ghosts(c2, b)
+    dt = stable_dt(q, nx, ny)
+    oracle.sweep(c1, a, 1, dt)
+    oracle.sweep(c2, b, 1, dt)
+    assert not np.array_equal(a, b)
+
+
+# ------------------------------------------------------ time controller KATs
+def _tc(oracle, **kw):
+    base = dict(t0=0.0, t_final=1.0, dt0=0.125, n_snap=4, cfl_target=0.75, dt_max=0.5,
+                dt_min=2.0 ** -20, growth=1.25, shrink=0.5, adaptive=True)
+    base.update(kw)
+    return oracle.TimeSettings(**base).tc()
+
+
+def _attempt(oracle, tc, cfl):
+    import ctypes as C
+
+    L = oracle.lib()
+    L.orc_tc_update_max_cfl(C.byref(tc), cfl)
+    ok = L.orc_tc_check_validity(C.byref(tc))
+    L.orc_tc_update_time(C.byref(tc), ok)
+    L.orc_tc_reset_cfl(C.byref(tc))
+    return ok
+
+
+# (CFL sequence) -> expected (valid, dt after, time after) per attempt; all
+# values are exact binary fractions
+KATS = {
+    "growth_clamp": ([0.25], [(1, 0.15625, 0.15625)]),          # cf = 3 -> 1.25
+    "deadband": ([0.7455078125], [(1, 0.125, 0.125)]),          # cf ~ 1.006 -> 1
+    "shrink": ([1.0], [(1, 0.09375, 0.09375)]),                  # cf = 0.75
+    "reject": ([2.0], [(0, 0.03125, 0.0)]),                       # 0.375 < 0.5 -> dt/4, time kept
+    "zero_cfl": ([0.0], [(1, 0.125, 0.125)]),                    # cf = 1
+    "snapshot_align": ([0.75, 0.75], [(1, 0.125, 0.125), (1, 0.125, 0.25)]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(KATS))
+def test_time_controller_kats(oracle, name):
+    cfls, expect = KATS[name]
+    tc = _tc(oracle)
+    for cfl, (v, dt, t) in zip(cfls, expect):
+        ok = _attempt(oracle, tc, cfl)
+        assert (ok, tc.dt, tc.time) == (v, dt, t)
+
+
+def test_time_controller_q12_lag_and_snapshot_cut(oracle):
+    """time advances by the NEXT dt (Q12) and the next dt is cut to land on
+    the snapshot time."""
+    tc = _tc(oracle, dt0=0.2, n_snap=4, cfl_target=0.75)  # snapshots 0.25, 0.5, 0.75, 1
+    ok = _attempt(oracle, tc, 0.25)  # cf 3 -> 1.25: 0.25 would pass the snapshot 0.25? no: 0+0.25 == 0.25
+    assert ok and tc.dt == 0.25 and tc.time == 0.25
+    tc2 = _tc(oracle, dt0=0.22)
+    _attempt(oracle, tc2, 0.25)      # 0.22*1.25 = 0.275 > 0.25 -> cut to 0.25
+    assert tc2.dt == 0.25 and tc2.time == 0.25 and tc2.snapshot_exception == 1
+
+
+def test_time_controller_bounds(oracle):
+    tc = _tc(oracle, dt0=0.45, dt_max=0.5)
+    _attempt(oracle, tc, 0.1)                    # 0.45*1.25 = 0.5625 -> cut to snapshot 0.25
+    assert tc.dt == 0.25
+    tc = _tc(oracle, dt0=2.0 ** -20, dt_min=2.0 ** -20)
+    _attempt(oracle, tc, 100.0)                  # reject: dt/4 -> clamped to dt_min
+    assert tc.dt == 2.0 ** -20 and tc.time == 0.0
+    with pytest.raises(ValueError):
+        _tc(oracle, dt_min=1.0, dt_max=0.5)      # end_program: invalid bounds
+
+
+def test_host_time_control_mirror_matches_oracle(oracle):
+    """The package's host-side TimeControl reproduces the oracle's controller
+    on a long random CFL sequence."""
+    from paraclaws_amd.timectl import TimeControl
+
+    rng = np.random.default_rng(4)
+    tc_o = _tc(oracle, t_final=3.0, n_snap=7, cfl_target=0.9, growth=1.2, dt_max=0.05,
+               dt0=1e-4, dt_min=1e-8)
+    tc_p = TimeControl(0.0, 3.0, 1e-4, 7, 0.9, 0.05, 1e-8, 1.2, 0.5, True)
+    for _ in range(400):
+        cfl = float(rng.choice([rng.uniform(0.5, 1.3), rng.uniform(0.88, 0.92), rng.uniform(0, 3)]))
+        ok_o = _attempt(oracle, tc_o, cfl)
+        tc_p.update_max_CFL(cfl)
+        ok_p = tc_p.check_timestep_validity()
+        tc_p.update_time(ok_p)
+        tc_p.reset_CFL()
+        assert (bool(ok_o), tc_o.dt, tc_o.time) == (ok_p, tc_p.dt, tc_p.time)
