@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-updates/s of the 2D shallow-water wave-propagation step on
+one MI355X.
+
+Workload (one GPU, N = 1): circular dam break on [-1,1]^2, 8192 x 8192 cells,
+reflective walls, HLL + MC limiter, CFL 0.9, adaptive dt from 1e-10, 200
+accepted steps.  A step is one accepted time step of the whole domain (xi
+sweep, eta sweep, source, ghost fill); a cell-update is one interior cell
+advanced by one accepted step.
+
+  value      FP32 fused engine, device-controlled run (graph-captured steps),
+             whole-job cell-updates/s with the state resident in HBM.
+  fp64_plain the plain FP64 path (one kernel per reference stage, HBM work
+             arrays) on the same workload; speedup = value / fp64_plain.
+  errors     per-field relative L1/Linf of FP32 against the FP64 path after
+             the same accepted steps, FP32 replaying the FP64 dt sequence.
+  roofline   the fused step kernel: 24 algorithmic bytes per cell-update
+             (read h,hu,hv once, write once, FP32) x cells per launch / its
+             average launch time from HIP events on the engine's stream.
+  cpu_baseline  the FP64 CPU restatement (OpenMP) on a bounded sample of the
+             same workload, on this machine's host cores.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_UPDATE_F32 = 24.0
+BYTES_PER_CELL_UPDATE_F64 = 48.0
+METRIC = "cell-updates/s/GPU (FP32 batched vs FP64 plain) at stated error; % HBM peak"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+TIME_KW = dict(t_final=0.5, dt0=1e-10, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10, growth=1.2,
+               shrink=0.5, n_snapshots=10)
+
+
+def run_engine(abi, ic, n, precision, engine, steps, warmup, spg, autotune, log_steps=0):
+    dom = abi.Domain(n, n, bc=(4, 4, 4, 4))
+    q0 = ic.dam_break(n, n)
+    e = abi.Engine([dom], precision=precision, engine=engine, limiter=4)
+    e.set_state(0, q0)
+    e.fill_ghosts()
+    e.time_init(**TIME_KW)
+    tuned = None
+    if autotune:
+        tuned = e.autotune(steps=3)
+        log(f"  autotune -> tile_rows={tuned[0]} waves_per_block={tuned[1]}")
+    if warmup:
+        e.run(steps_per_graph=spg, max_accepted=warmup)
+        e.set_state(0, q0)
+        e.fill_ghosts()
+        e.time_init(**TIME_KW)
+    if log_steps:
+        e.enable_log(log_steps)
+    sync()
+    t0 = time.perf_counter()
+    e.run(steps_per_graph=spg, max_accepted=steps)
+    sync()
+    wall = time.perf_counter() - t0
+    st = e.status(0)
+    avg_ms, launches = e.kernel_stats()
+    return e, dict(wall_s=wall, accepted=st.accepted, rejected=st.rejected, time=st.time,
+                   avg_step_ms=avg_ms, graph_ms=e.last_elapsed_ms(), tuned=tuned)
+
+
+def replay_fp32(abi, ic, n, dts):
+    """FP32 engine stepped with a given dt sequence (host control)."""
+    e = abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=abi.FP32, limiter=4)
+    e.set_state(0, ic.dam_break(n, n))
+    e.fill_ghosts()
+    for dt in dts:
+        e.step(dt)
+        e.accept()
+    return e
+
+
+def cpu_baseline(n, target_s=15.0):
+    """FP64 CPU restatement (OpenMP) on the same dam break; as many steps from
+    the initial state as fit in about target_s seconds (at least one)."""
+    from oracle import oracle as O
+
+    cores = int(os.environ.get("PCW_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    cfg = O.Config(nx=n, ny=n, limiter=4, bc=(4, 4, 4, 4), nthreads=cores)
+    ts = O.TimeSettings(**{"t_final": 0.5, "dt0": 1e-10, "cfl_target": 0.9, "dt_max": 1e-2,
+                           "dt_min": 1e-10, "growth": 1.2, "shrink": 0.5, "n_snap": 10})
+    q = O.dam_break(cfg)
+    t0 = time.perf_counter()
+    recs, tc = O.run(cfg, ts, q, max_accepted=3)
+    one = (time.perf_counter() - t0) / 3
+    steps = max(1, min(int(target_s / max(one, 1e-3)), 200))
+    q = O.dam_break(cfg)
+    t0 = time.perf_counter()
+    recs, tc = O.run(cfg, ts, q, max_accepted=steps)
+    el = time.perf_counter() - t0
+    acc = sum(r[0] for r in recs)
+    return dict(value=acc * n * n / el, unit="cell-updates/s", cores=cores, kind="port",
+                sample=f"dam break {n}x{n}, first {acc} accepted steps from the initial state, "
+                       f"FP64 CPU restatement with OpenMP ({el:.1f} s)")
+
+
+def load_traffic(n):
+    """HBM bytes per launch of the fused FP32 step from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs, if they exist for this grid size."""
+    p = os.path.join(ROOT, "profiles", f"pmc_fused_step_f32_{n}.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--steps-per-graph", type=int, default=50)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(how the domain would shard is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    from paraclaws_amd import abi, ic
+
+    n = args.n
+    cells = n * n
+    log(f"FP32 fused engine, dam break {n}^2, {args.steps} accepted steps")
+    e32, r32 = run_engine(abi, ic, n, abi.FP32, abi.FUSED, args.steps, args.warmup,
+                          args.steps_per_graph, not args.no_autotune)
+    value = r32["accepted"] * cells / r32["wall_s"]
+    log(f"  wall {r32['wall_s'] * 1e3:.2f} ms  {value:.4e} cell-updates/s  "
+        f"kernel avg {r32['avg_step_ms']:.4f} ms  rejected {r32['rejected']}")
+    kernel_ms = r32["avg_step_ms"]
+    achieved = BYTES_PER_CELL_UPDATE_F32 * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    traffic = load_traffic(n)
+    free_state = e32.get_state(0)
+    e32.close()
+
+    out_extra = {}
+    if not args.no_fp64:
+        log(f"FP64 plain (staged) engine, same workload")
+        e64, r64 = run_engine(abi, ic, n, abi.FP64, abi.STAGED, args.steps, min(args.warmup, 5),
+                              args.steps_per_graph, False, log_steps=args.steps + 64)
+        v64 = r64["accepted"] * cells / r64["wall_s"]
+        log(f"  wall {r64['wall_s'] * 1e3:.2f} ms  {v64:.4e} cell-updates/s")
+        dts = [r[2] for r in e64.read_log(0) if r[0] == 1]
+        ref = e64.get_state(0)
+        e64.close()
+        log(f"FP32 replay of the FP64 dt sequence ({len(dts)} steps) for the error")
+        er = replay_fp32(abi, ic, n, dts)
+        got = er.get_state(0)
+        er.close()
+        errs = ic.field_errors(got, ref)
+        log(f"  errors {errs}")
+        out_extra["fp64_plain"] = {"value": v64, "unit": "cell-updates/s",
+                                   "ms_per_step": r64["wall_s"] * 1e3 / max(1, r64["accepted"]),
+                                   "engine": "staged (one kernel per reference stage)"}
+        out_extra["speedup_vs_fp64_plain"] = value / v64
+        out_extra["errors_fp32_vs_fp64"] = errs
+        out_extra["free_running_time_fp32_vs_fp64"] = [r32["time"], r64["time"]]
+        del free_state, ref, got
+        log(f"FP64 fused engine (bit-exact mode), same workload")
+        ef, rf = run_engine(abi, ic, n, abi.FP64, abi.FUSED, args.steps, min(args.warmup, 5),
+                            args.steps_per_graph, False)
+        vf = rf["accepted"] * cells / rf["wall_s"]
+        ef.close()
+        log(f"  {vf:.4e} cell-updates/s")
+        out_extra["fp64_fused"] = {"value": vf, "unit": "cell-updates/s",
+                                   "roofline_frac_48B": vf * BYTES_PER_CELL_UPDATE_F64 / 1e9 / HBM_PEAK_GBS}
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (FP64 restatement, OpenMP)")
+        cpu = cpu_baseline(n)
+        log(f"  {cpu}")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": r32["accepted"],
+        "warmup": args.warmup,
+        "ms_per_step": r32["wall_s"] * 1e3 / max(1, r32["accepted"]),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"circular dam break {n}x{n} on [-1,1]^2, reflective walls, "
+                               f"HLL + MC limiter, CFL 0.9, adaptive dt from 1e-10, "
+                               f"{args.steps} accepted steps, FP32 fused engine",
+                   "cells": cells, "precision": "fp32 state/flux, fp64 dt/time",
+                   "tile_rows_waves": r32["tuned"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_fused_step<float>", "kernel_avg_ms": kernel_ms,
+                     "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_F32},
+        "cpu_baseline": cpu,
+    }
+    line.update(out_extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

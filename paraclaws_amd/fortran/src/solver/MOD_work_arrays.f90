@@ -1,0 +1,51 @@
+! work_arrays of the reference (src/solver/MOD_work_arrays.f90).  The wave,
+! fluctuation, limiter and correction arrays live on the device inside the
+! engine (registers of the fused step, or HBM planes of the staged engine);
+! this type keeps the type-bound interface and owns the engine's lifetime.
+module MOD_work_arrays
+    implicit none
+
+    type work_arrays
+        logical :: ready = .false.
+    contains
+        procedure :: allocate_work_arrays
+        procedure :: deallocate_work_arrays
+        procedure :: flash_work_to_zero
+        procedure :: initialize_work
+    end type
+
+contains
+
+    subroutine allocate_work_arrays(work, decomposition, mesh)
+        use MOD_decomposition, only: decomp_info
+        use MOD_domain, only: mesh_fields
+        use MOD_device, only: device_create
+        use MOD_solver_parameters
+        class(work_arrays), intent(inout) :: work
+        type(decomp_info), intent(in) :: decomposition
+        type(mesh_fields), intent(in) :: mesh
+        call device_create(mesh%m_xi, mesh%m_eta, &
+            [mesh%bc_xi_low, mesh%bc_xi_high, mesh%bc_eta_low, mesh%bc_eta_high], &
+            mesh%xi_dimensions, mesh%eta_dimensions, grav, device_precision, device_engine, &
+            add_higher_order_flux_correction, use_limiters, limiter_method)
+        work%ready = .true.
+    end subroutine
+
+    subroutine deallocate_work_arrays(work)
+        use MOD_device, only: device_destroy
+        class(work_arrays), intent(inout) :: work
+        call device_destroy()
+        work%ready = .false.
+    end subroutine
+
+    subroutine flash_work_to_zero(work)
+        !! the engine keeps no accumulating work arrays between sweeps
+        class(work_arrays), intent(inout) :: work
+    end subroutine
+
+    subroutine initialize_work(work)
+        class(work_arrays), intent(inout) :: work
+        call work%flash_work_to_zero()
+    end subroutine
+
+end module MOD_work_arrays

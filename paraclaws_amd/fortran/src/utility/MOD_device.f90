@@ -1,0 +1,102 @@
+! The rank's engine handle and the host<->device transfers of the solution.
+! The state lives on the GPU between steps; the host copy of
+! solution%vars is refreshed only at start-up, snapshots and user hooks.
+module MOD_device
+    use iso_c_binding
+    use MOD_pcw
+    implicit none
+
+    type(c_ptr), save :: handle = c_null_ptr
+    integer, save :: m_xi_ = 0, m_eta_ = 0
+
+contains
+
+    subroutine device_check(rc, what)
+        use MOD_error_handling, only: abort_program
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: what
+        if (rc < 0) call abort_program(what // ": " // pcw_error_message(handle))
+    end subroutine
+
+    subroutine device_create(m_xi, m_eta, bc, xi_dimensions, eta_dimensions, grav_, &
+            precision_, engine_, higher_order, use_limiter, limiter)
+        use MOD_constants, only: ud_precision
+        integer, intent(in) :: m_xi, m_eta, bc(4), precision_, engine_, limiter
+        double precision, intent(in) :: xi_dimensions(2), eta_dimensions(2), grav_
+        logical, intent(in) :: higher_order, use_limiter
+        type(pcw_params) :: p
+        type(pcw_domain) :: d(1)
+        integer(c_int) :: rc
+        p%precision = precision_
+        p%engine = engine_
+        p%higher_order = merge(1, 0, higher_order)
+        p%use_limiter = merge(1, 0, use_limiter)
+        p%limiter = limiter
+        p%tile_rows = 0
+        p%waves_per_block = 0
+        p%pad_ = 0
+        p%grav = grav_
+        p%eps = ud_precision
+        p%dry_tol = 0.d0
+        d(1)%nx = m_xi
+        d(1)%ny = m_eta
+        d(1)%bc = bc
+        d(1)%source = 0
+        d(1)%pad_ = 0
+        d(1)%xi_min = xi_dimensions(1)
+        d(1)%xi_max = xi_dimensions(2)
+        d(1)%eta_min = eta_dimensions(1)
+        d(1)%eta_max = eta_dimensions(2)
+        d(1)%manning_n = 0.d0
+        rc = pcw_create(p, d, 1_c_int32_t, handle)
+        call device_check(rc, "pcw_create")
+        m_xi_ = m_xi
+        m_eta_ = m_eta
+    end subroutine
+
+    subroutine device_upload(vars)
+        double precision, intent(in), contiguous, target :: vars(:,:,:)
+        call device_check(pcw_set_state(handle, 0_c_int32_t, vars), "pcw_set_state")
+    end subroutine
+
+    subroutine device_download(vars)
+        double precision, intent(inout), contiguous, target :: vars(:,:,:)
+        call device_check(pcw_get_state(handle, 0_c_int32_t, vars), "pcw_get_state")
+    end subroutine
+
+    subroutine device_fill_ghosts()
+        call device_check(pcw_fill_ghosts(handle), "pcw_fill_ghosts")
+    end subroutine
+
+    subroutine device_step(dt, smax_xi, smax_eta)
+        !! one attempt of both sweeps (+ source + ghost fill) into the pending state
+        double precision, intent(in) :: dt
+        double precision, intent(out) :: smax_xi, smax_eta
+        real(c_double) :: dts(1), smax(2)
+        dts(1) = dt
+        call device_check(pcw_step(handle, dts, smax), "pcw_step")
+        smax_xi = smax(1)
+        smax_eta = smax(2)
+    end subroutine
+
+    subroutine device_commit(keep)
+        logical, intent(in) :: keep
+        if (keep) then
+            call device_check(pcw_accept(handle), "pcw_accept")
+        else
+            call device_check(pcw_reject(handle), "pcw_reject")
+        end if
+    end subroutine
+
+    subroutine device_totals(out)
+        double precision, intent(out) :: out(3)
+        call device_check(pcw_totals(handle, 0_c_int32_t, out), "pcw_totals")
+    end subroutine
+
+    subroutine device_destroy()
+        integer(c_int) :: rc
+        if (c_associated(handle)) rc = pcw_destroy(handle)
+        handle = c_null_ptr
+    end subroutine
+
+end module MOD_device

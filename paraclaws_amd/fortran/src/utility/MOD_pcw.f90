@@ -1,0 +1,130 @@
+! iso_c_binding interfaces of the gfx950 engine (include/pcw.h).
+!
+! The derived types below are interoperable mirrors of pcw_params,
+! pcw_domain and pcw_time; every entry point returns the C status code
+! (0 ok, 1 rejected, < 0 fatal).
+module MOD_pcw
+    use iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: PCW_OK = 0, PCW_REJECTED = 1
+    integer(c_int), parameter :: PCW_FP64 = 0, PCW_FP32 = 1
+    integer(c_int), parameter :: PCW_ENGINE_FUSED = 0, PCW_ENGINE_STAGED = 1
+
+    type, bind(C) :: pcw_domain
+        integer(c_int32_t) :: nx, ny
+        integer(c_int32_t) :: bc(4)
+        integer(c_int32_t) :: source
+        integer(c_int32_t) :: pad_
+        real(c_double) :: xi_min, xi_max, eta_min, eta_max
+        real(c_double) :: manning_n
+    end type
+
+    type, bind(C) :: pcw_params
+        integer(c_int32_t) :: precision, engine
+        integer(c_int32_t) :: higher_order, use_limiter, limiter
+        integer(c_int32_t) :: tile_rows, waves_per_block, pad_
+        real(c_double) :: grav, eps, dry_tol
+    end type
+
+    interface
+        function pcw_create(p, domains, n, h) bind(C, name="pcw_create") result(rc)
+            import :: c_int, c_int32_t, c_ptr, pcw_params, pcw_domain
+            type(pcw_params), intent(in) :: p
+            type(pcw_domain), intent(in) :: domains(*)
+            integer(c_int32_t), value :: n
+            type(c_ptr), intent(out) :: h
+            integer(c_int) :: rc
+        end function
+
+        function pcw_destroy(h) bind(C, name="pcw_destroy") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int) :: rc
+        end function
+
+        function pcw_last_error(h) bind(C, name="pcw_last_error") result(msg)
+            import :: c_ptr
+            type(c_ptr), value :: h
+            type(c_ptr) :: msg
+        end function
+
+        function pcw_set_state(h, inst, vars) bind(C, name="pcw_set_state") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: inst
+            real(c_double), intent(in) :: vars(*)
+            integer(c_int) :: rc
+        end function
+
+        function pcw_get_state(h, inst, vars) bind(C, name="pcw_get_state") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: inst
+            real(c_double), intent(out) :: vars(*)
+            integer(c_int) :: rc
+        end function
+
+        function pcw_step(h, dt, smax) bind(C, name="pcw_step") result(rc)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(in) :: dt(*)
+            real(c_double), intent(out) :: smax(*)
+            integer(c_int) :: rc
+        end function
+
+        function pcw_accept(h) bind(C, name="pcw_accept") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int) :: rc
+        end function
+
+        function pcw_reject(h) bind(C, name="pcw_reject") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int) :: rc
+        end function
+
+        function pcw_fill_ghosts(h) bind(C, name="pcw_fill_ghosts") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int) :: rc
+        end function
+
+        function pcw_totals(h, inst, out) bind(C, name="pcw_totals") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: inst
+            real(c_double), intent(out) :: out(3)
+            integer(c_int) :: rc
+        end function
+    end interface
+
+contains
+
+    function pcw_error_message(h) result(msg)
+        !! The handle's last error as a Fortran string.
+        type(c_ptr), intent(in) :: h
+        character(len=:), allocatable :: msg
+        type(c_ptr) :: p
+        character(kind=c_char), pointer :: chars(:)
+        integer :: n
+        interface
+            function c_strlen(s) bind(C, name="strlen") result(len)
+                import :: c_ptr, c_size_t
+                type(c_ptr), value :: s
+                integer(c_size_t) :: len
+            end function
+        end interface
+        p = pcw_last_error(h)
+        if (.not. c_associated(p)) then
+            msg = "unknown engine error"
+            return
+        end if
+        n = int(c_strlen(p))
+        call c_f_pointer(p, chars, [n])
+        allocate(character(len=n) :: msg)
+        msg = transfer(chars(1:n), msg)
+    end function
+
+end module MOD_pcw

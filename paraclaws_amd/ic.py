@@ -1,0 +1,98 @@
+"""Initial conditions and synthetic inputs in the reference's frame layout.
+
+Frames are numpy arrays of shape (ny+4, nx+4, 3) holding
+solution%vars(3, -1:nx+2, -1:ny+2) (equation index fastest).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def centres(n: int, lo: float, hi: float) -> np.ndarray:
+    """Cell centres of frame indices -1..n+2, rounded as
+    src/domain/SMOD_mesh_manipulation.f90:50-51 computes them."""
+    d = (hi - lo) / n
+    i = np.arange(-1, n + 3, dtype=np.float64)
+    return (lo + (i - 1) * d) + d / 2
+
+
+def dam_break(nx: int, ny: int, xi=(-1.0, 1.0), eta=(-1.0, 1.0), h_in=2.0, h_out=1.0,
+              radius=0.5) -> np.ndarray:
+    """Circular dam break over the whole frame
+    (examples/2D_dam_break/cartesian/SMOD_set_initial_conditions.f90:34-46)."""
+    x = centres(nx, *xi)
+    y = centres(ny, *eta)
+    q = np.zeros((ny + 4, nx + 4, 3))
+    r = np.sqrt(x[None, :] * x[None, :] + y[:, None] * y[:, None])
+    q[..., 0] = np.where(r < radius, h_in, h_out)
+    return q
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014): a portable seeded generator so
+    synthetic inputs are reproducible from the seed alone."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.s = seed & self.MASK
+
+    def next_u64(self) -> int:
+        self.s = (self.s + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+
+def gaussian_bathymetry(nx, ny, seed, xi=(-1.0, 1.0), eta=(-1.0, 1.0), n_bumps=8,
+                        amp=(0.0, 0.8), width=(0.05, 0.3)) -> np.ndarray:
+    """b = sum_k A_k exp(-|x - x_k|^2 / (2 s_k^2)) over the frame (ny+4, nx+4)."""
+    g = SplitMix64(seed)
+    bumps = []
+    for _ in range(n_bumps):
+        a = g.uniform(*amp)
+        s = g.uniform(*width)
+        cx = g.uniform(*xi)
+        cy = g.uniform(*eta)
+        bumps.append((a, s, cx, cy))
+    x = centres(nx, *xi)[None, :]
+    y = centres(ny, *eta)[:, None]
+    b = np.zeros((ny + 4, nx + 4))
+    for a, s, cx, cy in bumps:
+        b += a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * s * s))
+    return b
+
+
+def still_water_with_step(b: np.ndarray, nx: int, xi=(-1.0, 1.0), level=1.0, step=1.0,
+                          dry_tol=1e-6) -> np.ndarray:
+    """h = max(0, level - b), dry below dry_tol, then h += step where x < 0;
+    hu = hv = 0."""
+    ny4, nx4 = b.shape
+    q = np.zeros((ny4, nx4, 3))
+    h = np.maximum(0.0, level - b)
+    h[h < dry_tol] = 0.0
+    x = centres(nx, *xi)[None, :]
+    h = h + np.where(x < 0.0, step, 0.0)
+    q[..., 0] = h
+    return q
+
+
+def field_errors(got: np.ndarray, ref: np.ndarray) -> dict:
+    """Per-field relative L1 and Linf over the interior; absolute values when
+    the reference field is identically zero."""
+    out = {}
+    g = got[2:-2, 2:-2]
+    r = ref[2:-2, 2:-2]
+    for k, name in enumerate(("h", "hu", "hv")):
+        d = np.abs(g[..., k] - r[..., k])
+        s1 = np.abs(r[..., k]).sum()
+        sm = np.abs(r[..., k]).max()
+        out[name] = {
+            "l1_rel": float(d.sum() / s1) if s1 > 0 else float(d.sum()),
+            "linf_rel": float(d.max() / sm) if sm > 0 else float(d.max()),
+        }
+    return out
