@@ -143,11 +143,27 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="FP32 fused run only (for rocprofv3 kernel traces and counters)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run a plain device copy of MIB MiB (FETCH/WRITE counter scale)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(how the domain would shard is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+
+        nel = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(nel, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_calibration} MiB 5 times")
+        return
 
     from paraclaws_amd import abi, ic
 
@@ -162,8 +178,11 @@ def main():
     kernel_ms = r32["avg_step_ms"]
     achieved = BYTES_PER_CELL_UPDATE_F32 * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
     traffic = load_traffic(n)
-    free_state = e32.get_state(0)
     e32.close()
+    if args.profile_only:
+        print(json.dumps({"value": value, "kernel_avg_ms": kernel_ms, "achieved_gbs": achieved,
+                          "tuned": r32["tuned"], "steps": r32["accepted"]}), flush=True)
+        return
 
     out_extra = {}
     if not args.no_fp64:
@@ -187,7 +206,7 @@ def main():
         out_extra["speedup_vs_fp64_plain"] = value / v64
         out_extra["errors_fp32_vs_fp64"] = errs
         out_extra["free_running_time_fp32_vs_fp64"] = [r32["time"], r64["time"]]
-        del free_state, ref, got
+        del ref, got
         log(f"FP64 fused engine (bit-exact mode), same workload")
         ef, rf = run_engine(abi, ic, n, abi.FP64, abi.FUSED, args.steps, min(args.warmup, 5),
                             args.steps_per_graph, False)
