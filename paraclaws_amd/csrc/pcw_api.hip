@@ -415,7 +415,6 @@ int pcw_set_bathymetry(pcw_handle* h, int32_t inst, const double* b) {
 // ------------------------------------------------- host-controlled stepping
 int pcw_step(pcw_handle* h, const double* dt, double* smax) {
   if (!h || !dt) return fail(h, PCW_ERR_ARG, "bad argument");
-  drop_graph(h);
   int rc = set_dts(h, dt, 0);
   if (rc) return rc;
   if (h->p.engine == PCW_ENGINE_FUSED) {
@@ -457,7 +456,6 @@ int pcw_sweep(pcw_handle* h, int32_t ixy, const double* dt, double* smax) {
   if (!h || !dt || (ixy != 1 && ixy != 2)) return fail(h, PCW_ERR_ARG, "bad argument");
   if (h->p.engine != PCW_ENGINE_STAGED)
     return fail(h, PCW_ERR_ARG, "single sweeps need the staged engine");
-  drop_graph(h);
   int rc = set_dts(h, dt, 0);
   if (rc) return rc;
   long long woff = 0;
@@ -480,7 +478,6 @@ int pcw_sweep(pcw_handle* h, int32_t ixy, const double* dt, double* smax) {
 
 int pcw_source(pcw_handle* h, const double* dt) {
   if (!h || !dt) return fail(h, PCW_ERR_ARG, "bad argument");
-  drop_graph(h);
   int rc = set_dts(h, dt, 0);
   if (rc) return rc;
   hipError_t e;
@@ -511,7 +508,6 @@ int pcw_fill_ghosts(pcw_handle* h) {
 int pcw_time_init(pcw_handle* h, const pcw_time* t) {
   if (!h || !t) return PCW_ERR_ARG;
   if (t->dt_min > t->dt_max) return fail(h, PCW_ERR_CONTROL, "ABORT: Invalid timestep bounds!");
-  drop_graph(h);
   int rc = pull_states(h);
   if (rc) return rc;
   const int ns = t->n_snapshots < 1 ? 1 : t->n_snapshots;
@@ -541,7 +537,6 @@ int pcw_time_init(pcw_handle* h, const pcw_time* t) {
 
 int pcw_enable_log(pcw_handle* h, int32_t capacity) {
   if (!h || capacity < 0) return PCW_ERR_ARG;
-  drop_graph(h);
   int rc = pull_states(h);
   if (rc) return rc;
   if (h->dlog) HIPCHK(hipFree(h->dlog));

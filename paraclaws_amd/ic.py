@@ -96,3 +96,18 @@ def field_errors(got: np.ndarray, ref: np.ndarray) -> dict:
             "linf_rel": float(d.max() / sm) if sm > 0 else float(d.max()),
         }
     return out
+
+
+def ensemble_config4(n_inst=512, seed=4, n_min=64, n_max=1024, ratio=(1.5, 10.0)):
+    """Batched dam-break ensemble: square sizes log-uniform in [n_min, n_max]
+    and inner depth uniform in ratio, from a splitmix64 stream of `seed`.
+    Returns a list of (n, depth_ratio)."""
+    import math
+
+    g = SplitMix64(seed)
+    out = []
+    for _ in range(n_inst):
+        n = int(round(math.exp(g.uniform(math.log(n_min), math.log(n_max)))))
+        r = g.uniform(*ratio)
+        out.append((n, r))
+    return out
