@@ -131,6 +131,90 @@ def run_ensemble(abi, ic, spg):
                 max_attempts=acc_max, tuned=tuned)
 
 
+def run_bathymetry(abi, ic, n, steps, spg):
+    """Config 3: FP32 n x n over a seeded Gaussian-sum bathymetry (seed 3),
+    still water h + b = 1 with dry cells (tol 1e-6), dam-break step h += 1 on
+    x < 0, outflow (zero-gradient) BCs; FP64 mass total before and after."""
+    b = ic.gaussian_bathymetry(n, n, seed=3)
+    q = ic.still_water_with_step(b, n, dry_tol=1e-6)
+    dom = abi.Domain(n, n, bc=(2, 2, 2, 2), source=abi.SRC_BATHY)
+    e = abi.Engine([dom], precision=abi.FP32, limiter=4, dry_tol=1e-6)
+    e.set_state(0, q)
+    e.set_bathymetry(0, b)
+    e.fill_ghosts()
+    del q, b
+    m0 = e.totals(0)[0]
+    e.time_init(**TIME_KW)
+    tuned = e.autotune(steps=3)
+    e.run(steps_per_graph=spg, max_accepted=10)
+    m_after_warm = None
+    e.set_state(0, ic.still_water_with_step(ic.gaussian_bathymetry(n, n, seed=3), n, dry_tol=1e-6))
+    e.fill_ghosts()
+    e.time_init(**TIME_KW)
+    sync()
+    t0 = time.perf_counter()
+    e.run(steps_per_graph=spg, max_accepted=steps)
+    sync()
+    wall = time.perf_counter() - t0
+    st = e.status(0)
+    m1 = e.totals(0)[0]
+    avg_ms, _ = e.kernel_stats()
+    e.close()
+    cells = n * n
+    return dict(value=st.accepted * cells / wall, unit="cell-updates/s", steps=st.accepted,
+                rejected=st.rejected, kernel_avg_ms=avg_ms,
+                roofline_frac_28B=(28.0 * cells / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if avg_ms else None,
+                mass_fp64_before=m0 * (2.0 / n) ** 2, mass_fp64_after=m1 * (2.0 / n) ** 2,
+                tuned=tuned)
+
+
+def run_friction_batch(abi, ic, spg, t_final=2.0, n_inst=64):
+    """Config 5: 64 instances of 2048 x 1024 on [-2,2]x[-1,1], Manning n
+    uniform in [0.01, 0.05] and Gaussian-sum bathymetry per instance (seed 5),
+    superbee, outflow in x, walls in y, FP32 state, FP64 totals, to t = 2.0."""
+    g = ic.SplitMix64(5)
+    nx, ny = 2048, 1024
+    doms, bs, ns = [], [], []
+    for k in range(n_inst):
+        nman = g.uniform(0.01, 0.05)
+        ns.append(nman)
+        doms.append(abi.Domain(nx, ny, bc=(2, 2, 4, 4), xi=(-2.0, 2.0), eta=(-1.0, 1.0),
+                               source=abi.SRC_BATHY_FRICTION, manning_n=nman))
+    e = abi.Engine(doms, precision=abi.FP32, limiter=2, dry_tol=1e-6)
+    m0 = 0.0
+    for k in range(n_inst):
+        b = ic.gaussian_bathymetry(nx, ny, seed=5000 + k, xi=(-2.0, 2.0), eta=(-1.0, 1.0))
+        q = ic.still_water_with_step(b, nx, xi=(-2.0, 2.0), dry_tol=1e-6)
+        e.set_state(k, q)
+        e.set_bathymetry(k, b)
+    e.fill_ghosts()
+    for k in range(n_inst):
+        m0 += e.totals(k)[0]
+    tkw = dict(TIME_KW)
+    tkw["t_final"] = t_final
+    e.time_init(**tkw)
+    tuned = e.autotune(steps=2)
+    e.time_init(**tkw)
+    sync()
+    t0 = time.perf_counter()
+    e.run(steps_per_graph=spg)
+    sync()
+    wall = time.perf_counter() - t0
+    upd = 0
+    steps = []
+    m1 = 0.0
+    for k in range(n_inst):
+        st = e.status(k)
+        upd += st.accepted * nx * ny
+        steps.append(st.accepted)
+        m1 += e.totals(k)[0]
+    e.close()
+    cell = (4.0 / nx) * (2.0 / ny)
+    return dict(value=upd / wall, unit="cell-updates/s", instances=n_inst, wall_s=wall,
+                steps_min=min(steps), steps_max=max(steps), mass_fp64_before=m0 * cell,
+                mass_fp64_after=m1 * cell, tuned=tuned)
+
+
 def cpu_baseline(n, target_s=15.0):
     """FP64 CPU restatement (OpenMP) on the same dam break; as many steps from
     the initial state as fit in about target_s seconds (at least one)."""
@@ -176,6 +260,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-physics", action="store_true",
+                    help="skip the bathymetry (8192^2) and friction-batch legs")
     ap.add_argument("--profile-only", action="store_true",
                     help="FP32 fused run only (for rocprofv3 kernel traces and counters)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -254,6 +340,16 @@ def main():
         ens = run_ensemble(abi, ic, 64)
         log(f"  {ens}")
         out_extra["batched_ensemble"] = ens
+
+    if not args.no_physics:
+        log(f"bathymetry + wet/dry, FP32 {n}^2, outflow, {args.steps} steps")
+        bat = run_bathymetry(abi, ic, n, args.steps, args.steps_per_graph)
+        log(f"  {bat}")
+        out_extra["bathymetry"] = bat
+        log("friction batch: 64 x 2048x1024, Manning + bathymetry, superbee, to t = 2.0")
+        fr = run_friction_batch(abi, ic, 64)
+        log(f"  {fr}")
+        out_extra["friction_batch"] = fr
 
     cpu = None
     if not args.no_cpu:

@@ -104,4 +104,10 @@ def test_fortran_host_run_matches_oracle(oracle, tmp_path, precision, engine):
     assert np.array_equal(got[m], q[m])
     snaps = sorted(os.listdir(tmp_path / "output"))
     assert "domain.dat" in snaps and "mesh_rank_0.dat" in snaps
-    assert sum(1 for s in snaps if s[0].isdigit()) == 11   # t=0 and 10 snapshots
+    times = sorted((s for s in snaps if s[0].isdigit()), key=float)
+    assert len(times) == 11   # t = 0 and 10 snapshots
+    # the last snapshot is the final state, written in the reference's
+    # qout.dat format: 3 values per line (50e26.16), j outer, i inner, ghosts included
+    qout = np.loadtxt(tmp_path / "output" / times[-1] / "rank_0" / "qout.dat")
+    assert qout.shape == ((n + 4) * (n + 4), 3)
+    assert np.allclose(qout.reshape(n + 4, n + 4, 3)[m], q[m], rtol=1e-15, atol=1e-16)
