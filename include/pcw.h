@@ -127,6 +127,13 @@ int pcw_set_state(pcw_handle* h, int32_t inst, const double* vars);
 int pcw_get_state(pcw_handle* h, int32_t inst, double* vars);
 /* bathymetry frame b(-1:nx+2, -1:ny+2), doubles */
 int pcw_set_bathymetry(pcw_handle* h, int32_t inst, const double* b);
+/* Mesh metrics of a mapped grid, frames (-1:nx+2, -1:ny+2) of doubles:
+ * mesh%quadrilateral_mapping%capacity, length_ratio_xi(1,:,:) and
+ * length_ratio_eta(1,:,:) (src/domain/SMOD_mesh_manipulation.f90:147-173).
+ * Without this call an instance uses the identity map's metrics, computed in
+ * the kernels with the reference's rounding. */
+int pcw_set_metrics(pcw_handle* h, int32_t inst, const double* kappa, const double* gamma_xi,
+                    const double* gamma_eta);
 
 /* ---- host-controlled stepping (the Fortran host's path) -----------------
  * pcw_step advances every instance by one attempt with its own dt into the

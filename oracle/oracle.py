@@ -36,6 +36,7 @@ class OrcCfg(C.Structure):
         ("eta_min", C.c_double), ("eta_max", C.c_double),
         ("grav", C.c_double), ("eps", C.c_double),
         ("manning_n", C.c_double), ("dry_tol", C.c_double),
+        ("kappa", C.c_void_p), ("gamma_xi", C.c_void_p), ("gamma_eta", C.c_void_p),
     ]
 
 
@@ -135,6 +136,10 @@ class Config:
     manning_n: float = 0.0
     dry_tol: float = 0.0
     nthreads: int = 0
+    # optional metric frames (ny+4, nx+4) of a mapped grid; None = identity map
+    kappa: object = None
+    gamma_xi: object = None
+    gamma_eta: object = None
 
     def c(self) -> OrcCfg:
         o = OrcCfg()
@@ -145,6 +150,12 @@ class Config:
         o.source, o.nthreads = self.source, self.nthreads
         o.xi_min, o.xi_max, o.eta_min, o.eta_max = self.xi_min, self.xi_max, self.eta_min, self.eta_max
         o.grav, o.eps, o.manning_n, o.dry_tol = self.grav, self.eps, self.manning_n, self.dry_tol
+        for name in ("kappa", "gamma_xi", "gamma_eta"):
+            a = getattr(self, name)
+            if a is not None:
+                assert a.dtype == np.float64 and a.flags.c_contiguous
+                assert a.shape == (self.ny + 4, self.nx + 4)
+                setattr(o, name, a.ctypes.data)
         return o
 
     def frame(self) -> np.ndarray:

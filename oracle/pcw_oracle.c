@@ -92,6 +92,20 @@ static inline metric_t metric_at(double xi_min, double d_xi, double eta_min,
   return m;
 }
 
+/* metrics of cell (i,j): the caller's arrays, or the identity map's */
+static inline double kap_of(const orc_cfg* c, const frame_t* f, double dx, double dy, int i, int j) {
+  if (c->kappa) return c->kappa[F(f, i, j)];
+  return metric_at(c->xi_min, dx, c->eta_min, dy, i, j).kappa;
+}
+static inline double gxi_of(const orc_cfg* c, const frame_t* f, double dx, double dy, int i, int j) {
+  if (c->gamma_xi) return c->gamma_xi[F(f, i, j)];
+  return metric_at(c->xi_min, dx, c->eta_min, dy, i, j).gamma_xi;
+}
+static inline double geta_of(const orc_cfg* c, const frame_t* f, double dx, double dy, int i, int j) {
+  if (c->gamma_eta) return c->gamma_eta[F(f, i, j)];
+  return metric_at(c->xi_min, dx, c->eta_min, dy, i, j).gamma_eta;
+}
+
 int orc_metrics(const orc_cfg* c, double* xi, double* eta, double* kappa,
                 double* gamma_xi, double* gamma_eta) {
   frame_t f = frame_of(c);
@@ -426,22 +440,22 @@ static int sweep_impl(const orc_cfg* c, work_t* w, double* vars, int ixy,
 #pragma omp parallel for schedule(static)
   for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i) {
-      metric_t m = metric_at(c->xi_min, d_xi, c->eta_min, d_eta, i, j);
+      const double kap = kap_of(c, f, d_xi, d_eta, i, j);
       double gl, gr;
       long kr;
       if (ixy == 1) {
-        gl = m.gamma_xi;
-        gr = metric_at(c->xi_min, d_xi, c->eta_min, d_eta, i + 1, j).gamma_xi;
+        gl = gxi_of(c, f, d_xi, d_eta, i, j);
+        gr = gxi_of(c, f, d_xi, d_eta, i + 1, j);
         kr = F(f, i + 1, j);
       } else {
-        gl = m.gamma_eta;
-        gr = metric_at(c->xi_min, d_xi, c->eta_min, d_eta, i, j + 1).gamma_eta;
+        gl = geta_of(c, f, d_xi, d_eta, i, j);
+        gr = geta_of(c, f, d_xi, d_eta, i, j + 1);
         kr = F(f, i, j + 1);
       }
       long k = F(f, i, j);
       for (int me = 0; me < 3; ++me)
         Q(vars, f, me, i, j) = Q(vars, f, me, i, j) -
-            dtod / m.kappa * (gl * w->Ap[3 * k + me] + gr * w->Am[3 * kr + me]);
+            dtod / kap * (gl * w->Ap[3 * k + me] + gr * w->Am[3 * kr + me]);
     }
 
   if (c->higher_order) {
@@ -498,8 +512,8 @@ static int sweep_impl(const orc_cfg* c, work_t* w, double* vars, int ixy,
       for (int j = 1; j <= ny; ++j)
         for (int i = 0; i <= nx + 1; ++i) {
           long k = F(f, i, j);
-          double ka = metric_at(c->xi_min, d_xi, c->eta_min, d_eta, i - 1, j).kappa;
-          double kb = metric_at(c->xi_min, d_xi, c->eta_min, d_eta, i, j).kappa;
+          double ka = kap_of(c, f, d_xi, d_eta, i - 1, j);
+          double kb = kap_of(c, f, d_xi, d_eta, i, j);
           for (int mw = 0; mw < 2; ++mw) {
             double al = fabs(LAM(w, k, mw));
             for (int me = 0; me < 3; ++me)
@@ -512,8 +526,8 @@ static int sweep_impl(const orc_cfg* c, work_t* w, double* vars, int ixy,
       for (int j = 0; j <= ny + 1; ++j)
         for (int i = 1; i <= nx; ++i) {
           long k = F(f, i, j);
-          double ka = metric_at(c->xi_min, d_xi, c->eta_min, d_eta, i, j - 1).kappa;
-          double kb = metric_at(c->xi_min, d_xi, c->eta_min, d_eta, i, j).kappa;
+          double ka = kap_of(c, f, d_xi, d_eta, i, j - 1);
+          double kb = kap_of(c, f, d_xi, d_eta, i, j);
           for (int mw = 0; mw < 2; ++mw) {
             double al = fabs(LAM(w, k, mw));
             for (int me = 0; me < 3; ++me)
@@ -526,7 +540,7 @@ static int sweep_impl(const orc_cfg* c, work_t* w, double* vars, int ixy,
 #pragma omp parallel for schedule(static)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
-        double kap = metric_at(c->xi_min, d_xi, c->eta_min, d_eta, i, j).kappa;
+        double kap = kap_of(c, f, d_xi, d_eta, i, j);
         long k = F(f, i, j);
         long kr = (ixy == 1) ? F(f, i + 1, j) : F(f, i, j + 1);
         for (int me = 0; me < 3; ++me)

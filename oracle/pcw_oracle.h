@@ -48,6 +48,13 @@ typedef struct orc_cfg {
   double eps;                     /* MOD_constants ud_precision (1e-10)       */
   double manning_n;               /* friction coefficient (source 2/3)        */
   double dry_tol;                 /* wet/dry threshold (source 1/3)           */
+  /* Optional mesh metrics of a mapped grid, frame arrays (nx+4)*(ny+4):
+   * capacity, length_ratio_xi(1,:,:), length_ratio_eta(1,:,:)
+   * (src/domain/SMOD_mesh_manipulation.f90:153,162,173).  NULL: the
+   * identity map's metrics are computed in place. */
+  const double* kappa;
+  const double* gamma_xi;
+  const double* gamma_eta;
 } orc_cfg;
 
 /* time_control (src/time_control/MOD_time_control.f90:16-66). */

@@ -74,6 +74,7 @@ SIGNATURES = {
     "pcw_set_state": ([_h, C.c_int32, _dp], C.c_int),
     "pcw_get_state": ([_h, C.c_int32, _dp], C.c_int),
     "pcw_set_bathymetry": ([_h, C.c_int32, _dp], C.c_int),
+    "pcw_set_metrics": ([_h, C.c_int32, _dp, _dp, _dp], C.c_int),
     "pcw_step": ([_h, _dp, _dp], C.c_int),
     "pcw_accept": ([_h], C.c_int),
     "pcw_reject": ([_h], C.c_int),
@@ -205,6 +206,10 @@ class Engine:
     def set_bathymetry(self, k, b):
         b = np.ascontiguousarray(b, dtype=np.float64)
         self._chk(self.L.pcw_set_bathymetry(self.h, k, _ptr(b)))
+
+    def set_metrics(self, k, kappa, gamma_xi, gamma_eta):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (kappa, gamma_xi, gamma_eta)]
+        self._chk(self.L.pcw_set_metrics(self.h, k, *[_ptr(a) for a in arrs]))
 
     # host-controlled stepping
     def step(self, dt):

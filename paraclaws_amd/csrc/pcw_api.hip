@@ -35,6 +35,9 @@ struct pcw_handle {
   long long pool_elems = 0;
   void* bpool = nullptr;
   long long bpool_elems = 0;
+  void* mpool = nullptr;          // metric planes of mapped-grid instances
+  long long mpool_elems = 0;
+  bool any_met = false, any_plain = true;
   double* dsnaps = nullptr;
   pcw_step_record* dlog = nullptr;
   int log_cap = 0;
@@ -129,24 +132,29 @@ void drop_graph(pcw_handle* h) {
 
 int limiter_code(const pcw_params& p) { return (p.higher_order && p.use_limiter) ? p.limiter : 0; }
 
-// enqueue one fused attempt for every instance
+// enqueue one fused attempt for every instance (instances with metric arrays
+// and those on the identity map run in separate launches of the kernel)
 int enqueue_fused(pcw_handle* h) {
   const int threads = 64 * h->wpb;
   int blocks = (h->ntiles + h->wpb - 1) / h->wpb;
   blocks = (int)round_up(blocks, 8);
-  hipError_t e;
-  if (h->p.precision == PCW_FP32) {
-    FusedArgs<float> a{(float*)h->pool, (const float*)h->bpool, h->dinst, h->dst, h->dtiles,
-                       h->ntiles, h->p.higher_order, h->p.dry_tol > 0.0, 0, h->p.grav,
-                       h->p.eps, h->p.dry_tol};
-    e = launch_fused_f32(limiter_code(h->p), a, blocks, threads, h->stream);
-  } else {
-    FusedArgs<double> a{(double*)h->pool, (const double*)h->bpool, h->dinst, h->dst, h->dtiles,
-                        h->ntiles, h->p.higher_order, h->p.dry_tol > 0.0, 0, h->p.grav,
-                        h->p.eps, h->p.dry_tol};
-    e = launch_fused_f64(limiter_code(h->p), a, blocks, threads, h->stream);
+  for (int met = 0; met <= 1; ++met) {
+    if (met ? !h->any_met : !h->any_plain) continue;
+    hipError_t e;
+    if (h->p.precision == PCW_FP32) {
+      FusedArgs<float> a{(float*)h->pool, (const float*)h->bpool, (const float*)h->mpool,
+                         h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
+                         h->p.dry_tol > 0.0, 0, h->p.grav, h->p.eps, h->p.dry_tol};
+      e = launch_fused_f32(limiter_code(h->p), met, a, blocks, threads, h->stream);
+    } else {
+      FusedArgs<double> a{(double*)h->pool, (const double*)h->bpool, (const double*)h->mpool,
+                          h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
+                          h->p.dry_tol > 0.0, 0, h->p.grav, h->p.eps, h->p.dry_tol};
+      e = launch_fused_f64(limiter_code(h->p), met, a, blocks, threads, h->stream);
+    }
+    if (e != hipSuccess)
+      return fail(h, PCW_ERR_HIP, std::string("fused launch: ") + hipGetErrorString(e));
   }
-  if (e != hipSuccess) return fail(h, PCW_ERR_HIP, std::string("fused launch: ") + hipGetErrorString(e));
   return PCW_OK;
 }
 
@@ -162,7 +170,8 @@ int enqueue_staged_sweeps(pcw_handle* h, int backup) {
   for (int k = 0; k < h->n; ++k) {
     const KInst& I = h->hinst[k];
     StagedArgs a{h->dinst + k, h->dst + k, (double*)h->pool + I.off[0], h->work + woff,
-                 h->p.grav, h->p.eps, h->p.dry_tol, limiter_code(h->p), 0};
+                 I.moff >= 0 ? (const double*)h->mpool + I.moff : nullptr, h->p.grav, h->p.eps,
+                 h->p.dry_tol, limiter_code(h->p), 0};
     for (int dir = 1; dir <= 2; ++dir) {
       hipError_t e = launch_staged_sweep(a, I, dir, h->p.higher_order, h->stream);
       if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
@@ -293,6 +302,7 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
     } else {
       I.boff = -1;
     }
+    I.moff = -1;
     h->max_nx = std::max(h->max_nx, d.nx);
     h->max_ny = std::max(h->max_ny, d.ny);
     h->max_side = std::max(h->max_side, d.nx + d.ny);
@@ -334,8 +344,8 @@ int pcw_destroy(pcw_handle* h) {
   if (!h) return PCW_ERR_ARG;
   drop_graph(h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* ptrs[] = {h->pool, h->bpool, h->dinst, h->dst, h->dtiles, h->dsnaps, h->dlog, h->work,
-                  h->dscratch};
+  void* ptrs[] = {h->pool, h->bpool, h->mpool, h->dinst, h->dst, h->dtiles, h->dsnaps, h->dlog,
+                  h->work, h->dscratch};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -350,7 +360,7 @@ int pcw_device_bytes(const pcw_handle* h, int64_t* bytes) {
   long long w = 0;
   if (h->work)
     for (const auto& I : h->hinst) w += (long long)W_COUNT * I.plane * 8;
-  *bytes = h->pool_elems * h->esize + h->bpool_elems * h->esize + w;
+  *bytes = (h->pool_elems + h->bpool_elems + h->mpool_elems) * h->esize + w;
   return PCW_OK;
 }
 
@@ -412,6 +422,45 @@ int pcw_set_bathymetry(pcw_handle* h, int32_t inst, const double* b) {
   return PCW_OK;
 }
 
+int pcw_set_metrics(pcw_handle* h, int32_t inst, const double* kappa, const double* gamma_xi,
+                    const double* gamma_eta) {
+  if (!h || !kappa || !gamma_xi || !gamma_eta || inst < 0 || inst >= h->n)
+    return fail(h, PCW_ERR_ARG, "bad argument");
+  if (!h->mpool) {  // one metric region per instance, allocated on first use
+    long long off = 0;
+    for (auto& I : h->hinst) off += 3 * I.plane;
+    HIPCHK(hipMalloc(&h->mpool, (size_t)off * h->esize));
+    HIPCHK(hipMemset(h->mpool, 0, (size_t)off * h->esize));
+    h->mpool_elems = off;
+  }
+  long long off = 0;
+  for (int k = 0; k < inst; ++k) off += 3 * h->hinst[k].plane;
+  KInst& I = h->hinst[inst];
+  I.moff = off;
+  const int nxf = I.nx + 4, nyf = I.ny + 4;
+  std::vector<char> buf((size_t)3 * I.plane * h->esize, 0);
+  const double* src[3] = {kappa, gamma_xi, gamma_eta};
+  for (int c = 0; c < 3; ++c)
+    for (int j = 0; j < nyf; ++j)
+      for (int i = 0; i < nxf; ++i) {
+        double v = src[c][(size_t)i + (size_t)nxf * j];
+        size_t o = (size_t)c * I.plane + (size_t)j * I.pitch + i;
+        if (h->esize == 4) ((float*)buf.data())[o] = (float)v;
+        else ((double*)buf.data())[o] = v;
+      }
+  HIPCHK(hipMemcpy((char*)h->mpool + (size_t)off * h->esize, buf.data(), buf.size(),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->dinst + inst, &I, sizeof(KInst), hipMemcpyHostToDevice));
+  h->any_met = false;
+  h->any_plain = false;
+  for (const auto& J : h->hinst) {
+    h->any_met = h->any_met || J.moff >= 0;
+    h->any_plain = h->any_plain || J.moff < 0;
+  }
+  drop_graph(h);  // the set of launches per step changed
+  return PCW_OK;
+}
+
 // ------------------------------------------------- host-controlled stepping
 int pcw_step(pcw_handle* h, const double* dt, double* smax) {
   if (!h || !dt) return fail(h, PCW_ERR_ARG, "bad argument");
@@ -462,7 +511,8 @@ int pcw_sweep(pcw_handle* h, int32_t ixy, const double* dt, double* smax) {
   for (int k = 0; k < h->n; ++k) {
     const KInst& I = h->hinst[k];
     StagedArgs a{h->dinst + k, h->dst + k, (double*)h->pool + I.off[0], h->work + woff,
-                 h->p.grav, h->p.eps, h->p.dry_tol, limiter_code(h->p), 0};
+                 I.moff >= 0 ? (const double*)h->mpool + I.moff : nullptr, h->p.grav, h->p.eps,
+                 h->p.dry_tol, limiter_code(h->p), 0};
     HIPCHK(launch_staged_sweep(a, I, ixy, h->p.higher_order, h->stream));
     woff += (long long)W_COUNT * I.plane;
   }

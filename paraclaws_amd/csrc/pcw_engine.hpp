@@ -21,6 +21,7 @@ struct KInst {
   long long plane;     // elements per field plane (pitch * (ny+4))
   long long off[2];    // element offset of the h-plane frame origin (i=-1,j=-1) of buffer 0/1
   long long boff;      // bathymetry frame offset, -1 when absent
+  long long moff;      // metric planes (kappa, gamma_xi, gamma_eta) offset, -1: identity map
   double dxi, deta, xi_min, eta_min;
   double manning;
 };

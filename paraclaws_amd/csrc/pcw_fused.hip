@@ -16,6 +16,8 @@
 // second_order_flux_correction, integrator_flux_correction_increment, the
 // CFL max (as max wave speed), integrate_source_term and
 // apply_physical_boundaries.
+#include <type_traits>
+
 #include "pcw_device.hpp"
 #include "pcw_engine.hpp"
 #include "pcw_kernels.hpp"
@@ -25,7 +27,7 @@ namespace pcw {
 template <typename T>
 __device__ __forceinline__ T ld(const T* p) { return *p; }
 
-template <typename T, bool EXACT, int LIM>
+template <typename T, bool EXACT, int LIM, bool MET>
 __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
@@ -40,6 +42,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   const KInst& I = a.inst[in];
   InstState* S = a.st + in;
   if (S->status != PCW_STATUS_RUNNING) return;
+  if (MET != (I.moff >= 0)) return;  // instances with and without metric arrays run in separate launches
 
   const int nx = I.nx, ny = I.ny, pitch = I.pitch;
   const long long plane = I.plane;
@@ -47,6 +50,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   const double dt = S->tc.dt;
   const T* __restrict__ qin = a.pool + I.off[cur];
   T* __restrict__ qout = a.pool + I.off[cur ^ 1];
+  const T* __restrict__ mq = MET ? a.mpool + I.moff : nullptr;  // kappa, gamma_xi, gamma_eta planes
   const int x0 = t.y, y0 = t.z, y1 = t.w;
   const int i = x0 - 2 + lane;                 // this lane's column (1-based interior)
   const int fc = i + 1 < nx + 3 ? i + 1 : nx + 3;  // frame column, clamped for loads
@@ -76,7 +80,9 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   const int l_nx = nx - x0 + 2;                // lane of column nx
   const bool do_left = x0 == 1;
   const bool do_right = l_nx >= 3 && l_nx <= 61;
-  const double gam_eta = EXACT ? gamma_eta_exact(M, i) : 1.0;
+  // metric scalars: MT is double in the exact mode, T otherwise
+  using MT = typename std::conditional<EXACT, double, T>::type;
+  const MT gam_eta_col = EXACT ? (MT)gamma_eta_exact(M, i) : MT(1);
 
   T smx = T(0), smy = T(0);
   int bad = 0;
@@ -90,7 +96,8 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   for (int k = 0; k < 3; ++k) { Wa.w1[k] = Wa.w2[k] = Wb.w1[k] = Wb.w2[k] = T(0); }
   T Fb[3] = {T(0), T(0), T(0)};       // correction flux at interface r-2
   T qpp[3] = {T(0), T(0), T(0)};      // row r-2 (xi-updated)
-  double kap_p = 1.0, kap_pp = 1.0;   // kappa(i, r-1), kappa(i, r-2)
+  MT kap_p = MT(1), kap_pp = MT(1);   // kappa(i, r-1), kappa(i, r-2)
+  MT gy_p = MT(1), gy_pp = MT(1);     // gamma_eta(i, r-1), gamma_eta(i, r-2) (metric arrays)
   T fin1[3] = {T(0), T(0), T(0)};     // first finished row (bottom ghosts)
   T finm[3] = {T(0), T(0), T(0)};     // previous finished row (top ghosts)
   T bm = T(0), b0 = T(0);             // bathymetry rows r-3, r-2 of this column
@@ -98,19 +105,32 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   // prefetch first row
   long long roff = (long long)(y0 - 2 + 1) * pitch + fc;
   T nh = ld(qin + roff), nhu = ld(qin + plane + roff), nhv = ld(qin + 2 * plane + roff);
+  T nk = T(1), ngx = T(1), ngy = T(1);
+  if (MET) {
+    nk = ld(mq + roff);
+    ngx = ld(mq + plane + roff);
+    ngy = ld(mq + 2 * plane + roff);
+  }
 
   for (int r = y0 - 2; r <= y1 + 2; ++r) {
     T q[3] = {nh, nhu, nhv};
+    const T mk_r = nk, mgx_r = ngx, mgy_r = ngy;
     if (r < y1 + 2) {  // prefetch the next row
       long long o = (long long)(r + 2) * pitch + fc;
       nh = ld(qin + o);
       nhu = ld(qin + plane + o);
       nhv = ld(qin + 2 * plane + o);
+      if (MET) {
+        nk = ld(mq + o);
+        ngx = ld(mq + plane + o);
+        ngy = ld(mq + 2 * plane + o);
+      }
     }
     T bnew = T(0);
     if (use_b && r >= y0) bnew = bb_[(long long)r * pitch + fc];  // b(i, r-1)
 
-    const double kap_r = EXACT ? kappa_exact(M, i, r) : 1.0;
+    const MT kap_r = MET ? (MT)mk_r : (EXACT ? (MT)kappa_exact(M, i, r) : MT(1));
+    const MT gy_r = MET ? (MT)mgy_r : gam_eta_col;
     T qx[3] = {q[0], q[1], q[2]};
 
     // ------------------------------------------------------ xi sweep
@@ -135,6 +155,10 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
       T w1R[3], F[3] = {T(0), T(0), T(0)};
 #pragma unroll
       for (int k = 0; k < 3; ++k) w1R[k] = shl1(w.w1[k]);
+      // metric neighbours (all lane shifts in wave-uniform control flow)
+      const MT kap_l = (EXACT || MET) ? shr1(kap_r) : MT(1);   // kappa(i-1, r)
+      const MT gx_l = MET ? (MT)mgx_r : (EXACT ? (MT)gamma_xi_exact(M, r) : MT(1));
+      const MT gx_r = MET ? shl1(gx_l) : gx_l;                  // gamma_xi(i+1, r)
       if (a.ho) {
         T w2L[3];
 #pragma unroll
@@ -150,9 +174,8 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
           }
         }
         if constexpr (EXACT) {
-          double ka = shr1(kap_r), kb = kap_r;
-          double al = w.s2;
-          double coef = 0.5 * al * (1.0 - al * dtod_x / (0.5 * (ka + kb)));
+          const double al = w.s2;
+          const double coef = 0.5 * al * (1.0 - al * dtod_x / (0.5 * (kap_l + kap_r)));
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
             double t1 = coef * (p1 * w.w1[k]);
@@ -160,26 +183,34 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
             F[k] = t1 + t2;
           }
         } else {
-          T coef = (T(0.5) * w.s2) * fmaf(-w.s2, dtdx, T(1));
+          T cfl = w.s2 * dtdx;
+          if (MET) cfl = cfl * __builtin_amdgcn_rcpf(T(0.5) * (kap_l + kap_r));
+          const T coef = (T(0.5) * w.s2) * (T(1) - cfl);
 #pragma unroll
           for (int k = 0; k < 3; ++k) F[k] = coef * fmaf(p1, w.w1[k], p2 * w.w2[k]);
         }
       }
-      // all lane shifts happen here, in wave-uniform control flow
       const T s2R = shl1(w.s2);
       T FR[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) FR[k] = shl1(F[k]);
       if (i >= 1 && i <= nx) {
         if constexpr (EXACT) {
-          const double gam = gamma_xi_exact(M, r);
           const double fac = dtod_x / kap_r;
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
             double ap = w.s2 * w.w2[k];
             double am = (-s2R) * w1R[k];
-            double q1 = q[k] - fac * (gam * ap + gam * am);
+            double q1 = q[k] - fac * (gx_l * ap + gx_r * am);
             qx[k] = a.ho ? q1 - dtod_x / kap_r * (FR[k] - F[k]) : q1;
+          }
+        } else if (MET) {
+          const T fac = dtdx * __builtin_amdgcn_rcpf(kap_r);
+          const T sl = gx_l * w.s2, sr = gx_r * s2R;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            T d = fmaf(sl, w.w2[k], -sr * w1R[k]) + (FR[k] - F[k]);
+            qx[k] = fmaf(-fac, d, q[k]);
           }
         } else {
 #pragma unroll
@@ -198,7 +229,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
       if (a.wetdry) cy = cell_fast<2, true>(qx[0], qx[1], qx[2], g, dry);
       else cy = cell_fast<2, false>(qx[0], qx[1], qx[2], g, dry);
     }
-    // the eta sweep reads rows [0, ny+1] of columns [1, nx]
+    // the eta sweep reads rows [-1, ny+1] of columns [1, nx]
     bad |= (cy.bad && out_lane && r <= ny + 1) ? 1 : 0;
     Waves<T> Wc;
     Wc.s2 = T(0);
@@ -222,8 +253,8 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
         }
       }
       if constexpr (EXACT) {
-        double al = Wa.s2;
-        double coef = 0.5 * al * (1.0 - al * dtod_y / (0.5 * (kap_pp + kap_p)));
+        const double al = Wa.s2;
+        const double coef = 0.5 * al * (1.0 - al * dtod_y / (0.5 * (kap_pp + kap_p)));
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           double t1 = coef * (p1 * Wa.w1[k]);
@@ -231,7 +262,9 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
           Fa[k] = t1 + t2;
         }
       } else {
-        T coef = (T(0.5) * Wa.s2) * fmaf(-Wa.s2, dtdy, T(1));
+        T cfl = Wa.s2 * dtdy;
+        if (MET) cfl = cfl * __builtin_amdgcn_rcpf(T(0.5) * (kap_pp + kap_p));
+        const T coef = (T(0.5) * Wa.s2) * (T(1) - cfl);
 #pragma unroll
         for (int k = 0; k < 3; ++k) Fa[k] = coef * fmaf(p1, Wa.w1[k], p2 * Wa.w2[k]);
       }
@@ -245,8 +278,16 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
         for (int k = 0; k < 3; ++k) {
           double ap = Wb.s2 * Wb.w2[k];
           double am = (-Wa.s2) * Wa.w1[k];
-          double q1 = qpp[k] - fac * (gam_eta * ap + gam_eta * am);
+          double q1 = qpp[k] - fac * (gy_pp * ap + gy_p * am);
           qn[k] = a.ho ? q1 - dtod_y / kap_pp * (Fa[k] - Fb[k]) : q1;
+        }
+      } else if (MET) {
+        const T fac = dtdy * __builtin_amdgcn_rcpf(kap_pp);
+        const T sl = gy_pp * Wb.s2, sr = gy_p * Wa.s2;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          T d = fmaf(sl, Wb.w2[k], -sr * Wa.w1[k]) + (Fa[k] - Fb[k]);
+          qn[k] = fmaf(-fac, d, qpp[k]);
         }
       } else {
 #pragma unroll
@@ -348,6 +389,8 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
     yc = cy;
     kap_pp = kap_p;
     kap_p = kap_r;
+    gy_pp = gy_p;
+    gy_p = gy_r;
     bm = b0;
     b0 = bnew;
   }
@@ -401,26 +444,28 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
 }
 
 // ------------------------------------------------------------- launcher
-template <typename T, bool EXACT>
+template <typename T, bool EXACT, bool MET>
 static hipError_t launch_lim(int lim, const FusedArgs<T>& a, int blocks, int threads,
                              hipStream_t s) {
   switch (lim) {
-    case 0: k_fused_step<T, EXACT, 0><<<blocks, threads, 0, s>>>(a); break;
-    case 1: k_fused_step<T, EXACT, 1><<<blocks, threads, 0, s>>>(a); break;
-    case 2: k_fused_step<T, EXACT, 2><<<blocks, threads, 0, s>>>(a); break;
-    case 3: k_fused_step<T, EXACT, 3><<<blocks, threads, 0, s>>>(a); break;
-    default: k_fused_step<T, EXACT, 4><<<blocks, threads, 0, s>>>(a); break;
+    case 0: k_fused_step<T, EXACT, 0, MET><<<blocks, threads, 0, s>>>(a); break;
+    case 1: k_fused_step<T, EXACT, 1, MET><<<blocks, threads, 0, s>>>(a); break;
+    case 2: k_fused_step<T, EXACT, 2, MET><<<blocks, threads, 0, s>>>(a); break;
+    case 3: k_fused_step<T, EXACT, 3, MET><<<blocks, threads, 0, s>>>(a); break;
+    default: k_fused_step<T, EXACT, 4, MET><<<blocks, threads, 0, s>>>(a); break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_fused_f32(int lim, const FusedArgs<float>& a, int blocks, int threads,
+hipError_t launch_fused_f32(int lim, bool met, const FusedArgs<float>& a, int blocks, int threads,
                             hipStream_t s) {
-  return launch_lim<float, false>(lim, a, blocks, threads, s);
+  return met ? launch_lim<float, false, true>(lim, a, blocks, threads, s)
+             : launch_lim<float, false, false>(lim, a, blocks, threads, s);
 }
-hipError_t launch_fused_f64(int lim, const FusedArgs<double>& a, int blocks, int threads,
+hipError_t launch_fused_f64(int lim, bool met, const FusedArgs<double>& a, int blocks, int threads,
                             hipStream_t s) {
-  return launch_lim<double, true>(lim, a, blocks, threads, s);
+  return met ? launch_lim<double, true, true>(lim, a, blocks, threads, s)
+             : launch_lim<double, true, false>(lim, a, blocks, threads, s);
 }
 
 }  // namespace pcw

@@ -25,6 +25,7 @@ struct StagedArgs {
   InstState* st;       // this instance's state (device)
   double* q;           // current buffer, h plane frame origin
   double* w;           // work planes
+  const double* m;     // metric planes of a mapped grid, or nullptr (identity map)
   double g, eps, dry;
   int limiter;         // 0 = unlimited, 1..4
   int pad;
@@ -34,6 +35,7 @@ template <typename T>
 struct FusedArgs {
   T* pool;
   const T* bpool;
+  const T* mpool;      // metric planes (instances with moff >= 0)
   const KInst* inst;
   InstState* st;
   const int4* tiles;   // (instance, x0, y0, y1): output columns x0..x0+59, rows y0..y1
@@ -44,10 +46,10 @@ struct FusedArgs {
   double g, eps, dry;
 };
 
-hipError_t launch_fused_f32(int lim, const FusedArgs<float>& a, int blocks, int threads,
-                            hipStream_t s);
-hipError_t launch_fused_f64(int lim, const FusedArgs<double>& a, int blocks, int threads,
-                            hipStream_t s);
+hipError_t launch_fused_f32(int lim, bool met, const FusedArgs<float>& a, int blocks,
+                            int threads, hipStream_t s);
+hipError_t launch_fused_f64(int lim, bool met, const FusedArgs<double>& a, int blocks,
+                            int threads, hipStream_t s);
 hipError_t launch_staged_sweep(const StagedArgs& a, const KInst& hI, int dir, int ho,
                                hipStream_t s);
 template <typename T>

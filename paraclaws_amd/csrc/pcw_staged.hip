@@ -21,6 +21,19 @@ __device__ __forceinline__ long long fidx(int pitch, int i, int j) {
   return (long long)(j + 1) * pitch + (i + 1);
 }
 
+// metrics of cell (i, j): metric planes of a mapped grid, else the identity map's
+__device__ __forceinline__ double st_kappa(const StagedArgs& a, const KInst& I, int i, int j) {
+  if (a.m) return a.m[fidx(I.pitch, i, j)];
+  Mesh M{I.xi_min, I.eta_min, I.dxi, I.deta};
+  return kappa_exact(M, i, j);
+}
+__device__ __forceinline__ double st_gamma(const StagedArgs& a, const KInst& I, int dir, int i,
+                                           int j) {
+  if (a.m) return a.m[(dir == 1 ? 1 : 2) * I.plane + fidx(I.pitch, i, j)];
+  Mesh M{I.xi_min, I.eta_min, I.dxi, I.deta};
+  return dir == 1 ? gamma_xi_exact(M, j) : gamma_eta_exact(M, i);
+}
+
 // interface (i, j) of sweep DIR, threads over the sweep's interface range
 template <int DIR>
 __device__ __forceinline__ bool iface_of(const KInst& I, int& i, int& j) {
@@ -102,13 +115,12 @@ __global__ void k_st_integrate(StagedArgs a, int dir) {
   if (S->status != PCW_STATUS_RUNNING) return;
   int i, j;
   if (!cell_of(I, i, j)) return;
-  Mesh M{I.xi_min, I.eta_min, I.dxi, I.deta};
   const double dt = S->tc.dt;
   const double dtod = dt / (dir == 1 ? I.dxi : I.deta);
-  const double kap = kappa_exact(M, i, j);
+  const double kap = st_kappa(a, I, i, j);
   double gl, gr;
-  if (dir == 1) { gl = gamma_xi_exact(M, j); gr = gamma_xi_exact(M, j); }
-  else { gl = gamma_eta_exact(M, i); gr = gamma_eta_exact(M, i); }
+  if (dir == 1) { gl = st_gamma(a, I, 1, i, j); gr = st_gamma(a, I, 1, i + 1, j); }
+  else { gl = st_gamma(a, I, 2, i, j); gr = st_gamma(a, I, 2, i, j + 1); }
   const long long pl = I.plane, k = fidx(I.pitch, i, j);
   const long long kr = (dir == 1) ? fidx(I.pitch, i + 1, j) : fidx(I.pitch, i, j + 1);
   for (int me = 0; me < 3; ++me) {
@@ -168,10 +180,9 @@ __global__ void k_st_second(StagedArgs a, int dir) {
   int i, j;
   bool ok = (dir == 1) ? iface_of<1>(I, i, j) : iface_of<2>(I, i, j);
   if (!ok) return;
-  Mesh M{I.xi_min, I.eta_min, I.dxi, I.deta};
   const double dtod = S->tc.dt / (dir == 1 ? I.dxi : I.deta);
-  const double ka = (dir == 1) ? kappa_exact(M, i - 1, j) : kappa_exact(M, i, j - 1);
-  const double kb = kappa_exact(M, i, j);
+  const double ka = (dir == 1) ? st_kappa(a, I, i - 1, j) : st_kappa(a, I, i, j - 1);
+  const double kb = st_kappa(a, I, i, j);
   const long long pl = I.plane, k = fidx(I.pitch, i, j);
   const double al = a.w[W_LAM * pl + k];
   const double coef = 0.5 * al * (1.0 - al * dtod / (0.5 * (ka + kb)));
@@ -188,9 +199,8 @@ __global__ void k_st_increment(StagedArgs a, int dir) {
   if (S->status != PCW_STATUS_RUNNING) return;
   int i, j;
   if (!cell_of(I, i, j)) return;
-  Mesh M{I.xi_min, I.eta_min, I.dxi, I.deta};
   const double dtod = S->tc.dt / (dir == 1 ? I.dxi : I.deta);
-  const double kap = kappa_exact(M, i, j);
+  const double kap = st_kappa(a, I, i, j);
   const long long pl = I.plane, k = fidx(I.pitch, i, j);
   const long long kr = (dir == 1) ? fidx(I.pitch, i + 1, j) : fidx(I.pitch, i, j + 1);
   for (int me = 0; me < 3; ++me) {

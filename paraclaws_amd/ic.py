@@ -111,3 +111,35 @@ def ensemble_config4(n_inst=512, seed=4, n_min=64, n_max=1024, ratio=(1.5, 10.0)
         r = g.uniform(*ratio)
         out.append((n, r))
     return out
+
+
+def mapped_metrics(nx, ny, mapping, xi=(-1.0, 1.0), eta=(-1.0, 1.0)):
+    """capacity and face length ratios of the mapped grid x, y = mapping(xi, eta),
+    as src/domain/SMOD_mesh_manipulation.f90:36-48 and :147-173 compute them:
+    corners in counter-clockwise order, length ratios of the faces 1->4 (xi
+    face) and 1->2 (eta face), shoelace area over the capacity.  Returns
+    (kappa, gamma_xi, gamma_eta, x, y), frames of shape (ny+4, nx+4); x, y are
+    the mapped cell centres."""
+    dxi = (xi[1] - xi[0]) / nx
+    deta = (eta[1] - eta[0]) / ny
+    i = np.arange(-1, nx + 3, dtype=np.float64)[None, :]
+    j = np.arange(-1, ny + 3, dtype=np.float64)[:, None]
+    xc1 = xi[0] + (i - 1) * dxi + 0 * j
+    xc2 = xc1 + dxi
+    ec1 = eta[0] + (j - 1) * deta + 0 * i
+    ec3 = ec1 + deta
+    cx = [xc1, xc2, xc2, xc1, xc1]
+    cy = [ec1, ec1, ec3, ec3, ec1]
+    px, py = zip(*[mapping(a, b) for a, b in zip(cx, cy)])
+    fdx, fdy = px[3] - px[0], py[3] - py[0]
+    gam_xi = np.sqrt(fdx * fdx + fdy * fdy) / deta
+    fdx, fdy = px[1] - px[0], py[1] - py[0]
+    gam_eta = np.sqrt(fdx * fdx + fdy * fdy) / dxi
+    area = np.zeros_like(xc1)
+    for m in range(4):
+        area = area + 0.5 * (py[m] + py[m + 1]) * (px[m] - px[m + 1])
+    kappa = area / (dxi * deta)
+    xcen = (xi[0] + (i - 1) * dxi) + dxi / 2 + 0 * j
+    ecen = (eta[0] + (j - 1) * deta) + deta / 2 + 0 * i
+    x, y = mapping(xcen, ecen)
+    return kappa, gam_xi, gam_eta, x, y

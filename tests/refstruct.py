@@ -204,9 +204,13 @@ def sweep(q, ixy, dt, nx, ny, g, ho=True, use_lim=True, lim=4, bounds=(-1.0, 1.0
 
 
 def solve(q, dt, nx, ny, g, cfl_target=0.9, shrink=0.5, **kw):
-    """Both sweeps with the validity test between them; returns (ierr, CFL)."""
+    """Both sweeps with the validity test between them; returns (ierr, CFL).
+    metrics=(kappa, gamma_xi, gamma_eta) frames replace the identity map's."""
     bounds = kw.pop("bounds", (-1.0, 1.0, -1.0, 1.0))
     met = _metrics(nx, ny, *bounds)
+    if kw.get("metrics") is not None:
+        k, gx, ge = kw["metrics"]
+        met = (met[0], met[1], k, gx, ge)
     work = Work(nx + 4, ny + 4)
     ho, use_lim, lim = kw.get("ho", True), kw.get("use_lim", True), kw.get("lim", 4)
     cfl = _sweep(q, 1, dt, nx, ny, met, work, g, ho, use_lim, lim, 0.0)

@@ -1,0 +1,130 @@
+"""Mapped grids (the reference's cartesian_stretching example and a curved
+map): capacity and length-ratio arrays through the oracle and the engine."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.helpers import rough_state, sides_mask, stable_dt
+
+STRETCH = lambda a, b: (4 * a, 2 * b)  # noqa: E731  x = 4 xi, y = 2 eta
+
+
+def CURVED(a, b):
+    return a + 0.08 * np.sin(np.pi * b), b + 0.05 * a * a
+
+
+def _cfg(oracle, nx, ny, mapping, **kw):
+    from paraclaws_amd import ic
+
+    k, gx, ge, x, y = ic.mapped_metrics(nx, ny, mapping)
+    cfg = oracle.Config(nx=nx, ny=ny, kappa=np.ascontiguousarray(k),
+                        gamma_xi=np.ascontiguousarray(gx), gamma_eta=np.ascontiguousarray(ge), **kw)
+    return cfg, (k, gx, ge, x, y)
+
+
+def test_identity_metric_arrays_equal_builtin(oracle):
+    """Passing the identity map's metrics as arrays changes nothing (bitwise)."""
+    nx, ny = 23, 17
+    ident = lambda a, b: (a, b)  # noqa: E731
+    cfg_a, _ = _cfg(oracle, nx, ny, ident, limiter=3, bc=(3, 4, 2, 2))
+    cfg_b = oracle.Config(nx=nx, ny=ny, limiter=3, bc=(3, 4, 2, 2))
+    m = oracle.metrics(cfg_b)
+    assert np.array_equal(m["kappa"], cfg_a.kappa)
+    assert np.array_equal(m["gamma_xi"], cfg_a.gamma_xi)
+    assert np.array_equal(m["gamma_eta"], cfg_a.gamma_eta)
+    q = rough_state(nx, ny, seed=4)
+    oracle.fill_ghosts(cfg_b, q)
+    a, b = q.copy(), q.copy()
+    dt = stable_dt(q, nx, ny)
+    oracle.sweep(cfg_a, a, 1, dt)
+    oracle.sweep(cfg_b, b, 1, dt)
+    assert np.array_equal(a, b)
+
+
+def test_stretched_metrics_closed_form():
+    from paraclaws_amd import ic
+
+    k, gx, ge, x, y = ic.mapped_metrics(20, 20, STRETCH)
+    assert np.allclose(k, 8.0, rtol=1e-14)
+    assert np.allclose(gx, 2.0, rtol=1e-14) and np.allclose(ge, 4.0, rtol=1e-14)
+
+
+@pytest.mark.parametrize("mapping", [STRETCH, CURVED], ids=["stretch", "curved"])
+def test_mapped_oracle_matches_reference_structured(oracle, mapping):
+    from tests import refstruct
+    from tests.helpers import GRAV
+
+    nx, ny = 9, 8
+    cfg, (k, gx, ge, _, _) = _cfg(oracle, nx, ny, mapping, limiter=4, bc=(2, 3, 4, 2))
+    q = rough_state(nx, ny, seed=6)
+    oracle.fill_ghosts(cfg, q)
+    dt = 0.5 * stable_dt(q, nx, ny)
+    a, b = q.copy(), q.copy()
+    tc = oracle.TimeSettings(dt0=dt, shrink=1e-9).tc()
+    assert oracle.solve(cfg, tc, a) == 1
+    rc, _ = refstruct.solve(b, dt, nx, ny, GRAV, shrink=1e-9, metrics=(k, gx, ge))
+    assert rc == 1
+    assert np.allclose(a, b, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+@pytest.mark.parametrize("mapping", [STRETCH, CURVED], ids=["stretch", "curved"])
+def test_gpu_fp64_mapped_bitwise(oracle, engine, mapping):
+    from paraclaws_amd import abi
+
+    nx, ny = 75, 66
+    cfg, (k, gx, ge, _, _) = _cfg(oracle, nx, ny, mapping, limiter=4, bc=(4, 3, 2, 4))
+    q = rough_state(nx, ny, seed=8)
+    oracle.fill_ghosts(cfg, q)
+    dt = 0.5 * stable_dt(q, nx, ny)
+    ref = q.copy()
+    tc = oracle.TimeSettings(dt0=dt, shrink=1e-9).tc()
+    assert oracle.solve(cfg, tc, ref) == 1
+    oracle.fill_ghosts(cfg, ref)
+    print(f"gpu fp64 mapped {engine}", flush=True)
+    with abi.Engine([abi.Domain(nx, ny, bc=cfg.bc)], precision=abi.FP64,
+                    engine=abi.FUSED if engine == "fused" else abi.STAGED, limiter=4) as e:
+        e.set_state(0, q)
+        e.set_metrics(0, k, gx, ge)
+        smax = e.step(dt)[0]
+        e.accept()
+        got = e.get_state(0)
+    m = sides_mask(nx, ny)
+    assert np.array_equal(got[m], ref[m])
+    assert max(smax[0] * dt / (2 / nx), smax[1] * dt / (2 / ny)) == tc.CFL
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_batch_mapped_and_plain(oracle):
+    """One handle with a mapped instance and an identity-map instance (two
+    kernel variants per step) equals each instance run alone."""
+    from paraclaws_amd import abi
+
+    n = 48
+    cfg_m, (k, gx, ge, _, _) = _cfg(oracle, n, n, STRETCH, limiter=4, bc=(4, 4, 4, 4))
+    qa = oracle.dam_break(oracle.Config(nx=n, ny=n))
+    qb = oracle.dam_break(oracle.Config(nx=n, ny=n), h_in=3.0)
+    tkw = dict(t_final=0.05, dt0=1e-4, n_snapshots=2)
+    for prec in (abi.FP64, abi.FP32):
+        print(f"gpu mixed batch precision={prec}", flush=True)
+        with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4)), abi.Domain(n, n, bc=(4, 4, 4, 4))],
+                        precision=prec, limiter=4) as e:
+            e.set_state(0, qa)
+            e.set_state(1, qb)
+            e.set_metrics(0, k, gx, ge)
+            e.fill_ghosts()
+            e.time_init(**tkw)
+            e.run(steps_per_graph=8)
+            both = [e.get_state(0), e.get_state(1)]
+        for idx, (q0, met) in enumerate(((qa, True), (qb, False))):
+            with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=prec, limiter=4) as e1:
+                e1.set_state(0, q0)
+                if met:
+                    e1.set_metrics(0, k, gx, ge)
+                e1.fill_ghosts()
+                e1.time_init(**tkw)
+                e1.run(steps_per_graph=8)
+                solo = e1.get_state(0)
+            assert np.array_equal(both[idx], solo), (prec, idx)
