@@ -19,7 +19,7 @@ contains
     subroutine allocate_work_arrays(work, decomposition, mesh)
         use MOD_decomposition, only: decomp_info
         use MOD_domain, only: mesh_fields
-        use MOD_device, only: device_create
+        use MOD_device, only: device_create, device_set_metrics
         use MOD_solver_parameters
         class(work_arrays), intent(inout) :: work
         type(decomp_info), intent(in) :: decomposition
@@ -28,8 +28,23 @@ contains
             [mesh%bc_xi_low, mesh%bc_xi_high, mesh%bc_eta_low, mesh%bc_eta_high], &
             mesh%xi_dimensions, mesh%eta_dimensions, grav, device_precision, device_engine, &
             add_higher_order_flux_correction, use_limiters, limiter_method)
+        ! a user coordinate map other than the identity: hand the engine the
+        ! capacity and length ratios computed from it (SMOD_mesh_manipulation)
+        if (.not. identity_map(mesh)) then
+            call device_set_metrics(mesh%quadrilateral_mapping%capacity, &
+                mesh%quadrilateral_mapping%length_ratio_xi(1,:,:), &
+                mesh%quadrilateral_mapping%length_ratio_eta(1,:,:))
+            write(*,*) "mapped grid: capacity and length ratios uploaded to the engine"
+        end if
         work%ready = .true.
     end subroutine
+
+    logical function identity_map(mesh)
+        use MOD_domain, only: mesh_fields
+        type(mesh_fields), intent(in) :: mesh
+        identity_map = all(mesh%physical_space%x_corner == mesh%computational_space%xi_corner) .and. &
+            all(mesh%physical_space%y_corner == mesh%computational_space%eta_corner)
+    end function
 
     subroutine deallocate_work_arrays(work)
         use MOD_device, only: device_destroy

@@ -64,6 +64,13 @@ contains
         call device_check(pcw_get_state(handle, 0_c_int32_t, vars), "pcw_get_state")
     end subroutine
 
+    subroutine device_set_metrics(kappa, gamma_xi, gamma_eta)
+        !! capacity and face length ratios of a mapped grid (frames incl. ghosts)
+        double precision, intent(in), contiguous :: kappa(:,:), gamma_xi(:,:), gamma_eta(:,:)
+        call device_check(pcw_set_metrics(handle, 0_c_int32_t, kappa, gamma_xi, gamma_eta), &
+            "pcw_set_metrics")
+    end subroutine
+
     subroutine device_fill_ghosts()
         call device_check(pcw_fill_ghosts(handle), "pcw_fill_ghosts")
     end subroutine

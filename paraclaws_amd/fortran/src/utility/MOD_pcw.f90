@@ -65,6 +65,15 @@ module MOD_pcw
             integer(c_int) :: rc
         end function
 
+        function pcw_set_metrics(h, inst, kappa, gamma_xi, gamma_eta) &
+                bind(C, name="pcw_set_metrics") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: inst
+            real(c_double), intent(in) :: kappa(*), gamma_xi(*), gamma_eta(*)
+            integer(c_int) :: rc
+        end function
+
         function pcw_step(h, dt, smax) bind(C, name="pcw_step") result(rc)
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h

@@ -111,3 +111,35 @@ def test_fortran_host_run_matches_oracle(oracle, tmp_path, precision, engine):
     qout = np.loadtxt(tmp_path / "output" / times[-1] / "rank_0" / "qout.dat")
     assert qout.shape == ((n + 4) * (n + 4), 3)
     assert np.allclose(qout.reshape(n + 4, n + 4, 3)[m], q[m], rtol=1e-15, atol=1e-16)
+
+
+@pytest.mark.gpu
+def test_fortran_host_stretched_grid_matches_oracle(oracle, tmp_path):
+    """The reference's stretched example (x = 4 xi, y = 2 eta, through a
+    USER_DIR override of SMOD_coordinate_mapping): the host uploads the
+    capacity and length ratios and the run matches the oracle given the same
+    metric arrays, bit for bit."""
+    from paraclaws_amd import ic
+
+    exe = os.path.join(FDIR, "examples", "dam_break_stretched", "build", "paraclaws_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("stretched example not built (run __graft_entry__.build())")
+    n = 50
+    (tmp_path / "control.in").write_text(_control_in(n, 0.05, bc=(2, 2, 2, 2)))
+    (tmp_path / "custom.in").write_text("0\n0\n")
+    print("fortran host stretched grid", flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "mapped grid" in r.stdout
+    got = np.fromfile(tmp_path / "output" / "final_state.bin", dtype=np.float64).reshape(n + 4, n + 4, 3)
+    k, gx, ge, x, y = ic.mapped_metrics(n, n, lambda a, b: (4 * a, 2 * b))
+    cfg = oracle.Config(nx=n, ny=n, limiter=4, bc=(2, 2, 2, 2), kappa=np.ascontiguousarray(k),
+                        gamma_xi=np.ascontiguousarray(gx), gamma_eta=np.ascontiguousarray(ge))
+    q = cfg.frame()
+    q[..., 0] = np.where(np.sqrt(x ** 2 + y ** 2) < 0.5, 2.0, 1.0)
+    ts = oracle.TimeSettings(t_final=0.05, cfl_target=0.9, dt_max=1e-2, n_snap=10)
+    oracle.run(cfg, ts, q)
+    from tests.helpers import sides_mask
+
+    m = sides_mask(n, n)
+    assert np.array_equal(got[m], q[m])
