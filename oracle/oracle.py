@@ -135,7 +135,8 @@ class Config:
     source: int = 0
     manning_n: float = 0.0
     dry_tol: float = 0.0
-    nthreads: int = 0
+    # OpenMP threads: the machine's CPU share for one GPU (16) at most
+    nthreads: int = min(16, os.cpu_count() or 1)
     # optional metric frames (ny+4, nx+4) of a mapped grid; None = identity map
     kappa: object = None
     gamma_xi: object = None

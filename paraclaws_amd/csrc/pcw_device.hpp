@@ -132,7 +132,7 @@ __device__ __forceinline__ CellQ<float> cell_fast(float h, float hu, float hv, f
     v = d ? 0.0f : v;
     gc = gh > 0.0f ? gh : 0.0f;
   }
-  float c = __builtin_sqrtf(gc);
+  float c = __builtin_amdgcn_sqrtf(gc);  // v_sqrt_f32 (1 ulp), not the correctly rounded expansion
   float un = (DIR == 1) ? u : v;
   r.s = fabsf(un) + c;
   if (DIR == 1) {
