@@ -43,6 +43,9 @@ extern "C" {
 #define PCW_ERR_STATE (-3)   /* NaN Jacobian / complex eigenvalues (reference error stop) */
 #define PCW_ERR_BC (-4)      /* BC code 1 (user) or unknown: abort_program */
 #define PCW_ERR_CONTROL (-5) /* end_program reached by the time controller */
+#define PCW_ERR_STATE_ETA (-6) /* pcw_step/pcw_sweep: NaN/complex eigenvalues in the eta sweep
+                                  only; fatal unless the xi sweep alone rejects the step (the
+                                  reference then never runs the eta sweep) */
 
 #define PCW_FP64 0           /* reference arithmetic order, bit-exact with the FP64 CPU restatement */
 #define PCW_FP32 1           /* FP32 state and flux arithmetic, FP64 time/dt/totals */

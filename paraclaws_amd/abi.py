@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libpcw_hip.so")
 
 PCW_OK, PCW_REJECTED = 0, 1
 PCW_ERR_ARG, PCW_ERR_HIP, PCW_ERR_STATE, PCW_ERR_BC, PCW_ERR_CONTROL = -1, -2, -3, -4, -5
+PCW_ERR_STATE_ETA = -6
 FP64, FP32 = 0, 1
 FUSED, STAGED = 0, 1
 BC_ZERO_GRADIENT, BC_CONSTANT_GRADIENT, BC_REFLECTIVE = 2, 3, 4
@@ -215,8 +216,15 @@ class Engine:
     def step(self, dt):
         dts = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.n,)))
         smax = np.zeros(2 * self.n)
-        self._chk(self.L.pcw_step(self.h, _ptr(dts), _ptr(smax)))
-        return smax.reshape(self.n, 2)
+        rc = self.L.pcw_step(self.h, _ptr(dts), _ptr(smax))
+        self._last_smax = smax.reshape(self.n, 2)
+        self._chk(rc)
+        return self._last_smax
+
+    def last_smax(self):
+        """Sweep maxima of the last pcw_step (also set when it reports an
+        eta-sweep error)."""
+        return self._last_smax
 
     def accept(self):
         self._chk(self.L.pcw_accept(self.h))

@@ -230,6 +230,22 @@ int check_errors(pcw_handle* h) {
   return PCW_OK;
 }
 
+// host control: NaN/complex eigenvalues of the last attempt, by sweep
+int step_bad_code(pcw_handle* h) {
+  int eta_only = 0;
+  for (int k = 0; k < h->n; ++k) {
+    if (h->hst[k].last_bad & 1)
+      return fail(h, PCW_ERR_STATE, "instance " + std::to_string(k) +
+                                        ": NaN in the Jacobian or complex eigenvalues in the xi sweep");
+    if (h->hst[k].last_bad & 2) eta_only = 1;
+  }
+  if (eta_only)
+    return fail(h, PCW_ERR_STATE_ETA,
+                "NaN in the Jacobian or complex eigenvalues in the eta sweep (fatal unless the xi "
+                "sweep alone rejects the step)");
+  return PCW_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -483,7 +499,7 @@ int pcw_step(pcw_handle* h, const double* dt, double* smax) {
       smax[2 * k] = h->hst[k].last_smax_x;
       smax[2 * k + 1] = h->hst[k].last_smax_y;
     }
-  return PCW_OK;
+  return step_bad_code(h);
 }
 
 int pcw_accept(pcw_handle* h) {
@@ -523,6 +539,10 @@ int pcw_sweep(pcw_handle* h, int32_t ixy, const double* dt, double* smax) {
   if (rc) return rc;
   if (smax)
     for (int k = 0; k < h->n; ++k) smax[k] = ixy == 1 ? h->hst[k].last_smax_x : h->hst[k].last_smax_y;
+  for (int k = 0; k < h->n; ++k)
+    if (h->hst[k].last_bad)
+      return fail(h, PCW_ERR_STATE, "instance " + std::to_string(k) +
+                                        ": NaN in the Jacobian or complex eigenvalues");
   return PCW_OK;
 }
 

@@ -48,7 +48,8 @@ struct InstState {
   int cur;                            // index of the current buffer
   int status;                         // PCW_STATUS_*
   int error;                          // PCW_ERR_* when status is ERROR
-  int bad;                            // set by tiles on a NaN Jacobian / complex eigenvalue
+  int bad;                            // set by tiles: bit 1 NaN/complex eigenvalue in the xi sweep, bit 2 in the eta sweep
+  int last_bad;                       // bits of the last completed attempt
   int controlled;                     // 1: the last tile runs the controller
   int last_ok;                        // staged engine: 1 when the last attempt was kept
   int log_n, log_cap;
@@ -168,6 +169,34 @@ __host__ __device__ inline int tc_attempt(TimeCtl& t, double smax_x, double smax
     rec->dt_next = t.dt;
   }
   return valid;
+}
+
+// Error semantics of one attempt with the device controller.  The reference
+// stops on a NaN Jacobian or complex eigenvalues of a sweep it runs
+// (SMOD_jakobian.f90:55-70, SMOD_eigenvalue_problem.f90:258-268), but it skips
+// the eta sweep when the xi sweep alone already rejects the step
+// (MOD_solve.f90:109-120).  Returns 1 when the instance stopped with an
+// error; else *ok is the accept decision (tc_attempt).
+__host__ __device__ inline int finish_checks(InstState* S, double sx, double sy, double dxi,
+                                             double deta, int badbits, pcw_step_record* rec,
+                                             int* ok) {
+  if (badbits & 1) {
+    S->status = PCW_STATUS_ERROR;
+    S->error = PCW_ERR_STATE;
+    return 1;
+  }
+  if (badbits & 2) {
+    // is the step rejected by the xi sweep alone?
+    TimeCtl probe = S->tc;
+    tc_update_max_cfl(probe, sx * probe.dt / dxi);
+    if (tc_valid(probe)) {
+      S->status = PCW_STATUS_ERROR;
+      S->error = PCW_ERR_STATE;
+      return 1;
+    }
+  }
+  *ok = tc_attempt(S->tc, sx, sy, dxi, deta, rec);
+  return 0;
 }
 
 }  // namespace pcw

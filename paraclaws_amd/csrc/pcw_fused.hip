@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   const MT gam_eta_col = EXACT ? (MT)gamma_eta_exact(M, i) : MT(1);
 
   T smx = T(0), smy = T(0);
-  int bad = 0;
+  int bad = 0;   // bit 1: NaN/complex in the xi sweep, bit 2: in the eta sweep
 
   // eta-sweep register window
   T yq[3] = {T(0), T(0), T(0)};      // row r-1 (xi-updated)
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
       else cy = cell_fast<2, false>(qx[0], qx[1], qx[2], g, dry);
     }
     // the eta sweep reads rows [-1, ny+1] of columns [1, nx]
-    bad |= (cy.bad && out_lane && r <= ny + 1) ? 1 : 0;
+    bad |= (cy.bad && out_lane && r <= ny + 1) ? 2 : 0;
     Waves<T> Wc;
     Wc.s2 = T(0);
 #pragma unroll
@@ -397,11 +397,11 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
 
   // ------------------------------------------------ reductions + controller
   const T mx = wave_max(smx), my = wave_max(smy);
-  const int anybad = __any(bad);
+  const int bad_x = __any(bad & 1), bad_y = __any(bad & 2);
   if (lane == 0) {
     atomicMax(&S->smax_x, (unsigned long long)__double_as_longlong((double)mx));
     atomicMax(&S->smax_y, (unsigned long long)__double_as_longlong((double)my));
-    if (anybad) atomicOr(&S->bad, 1);
+    if (bad_x | bad_y) atomicOr(&S->bad, (bad_x ? 1 : 0) | (bad_y ? 2 : 0));
   }
   __threadfence();
   unsigned int prev = 0;
@@ -418,14 +418,13 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   S->last_smax_x = sx;
   S->last_smax_y = sy;
   S->tiles_done = 0u;
-  if (atomicAdd(&S->bad, 0) != 0) {
-    S->status = PCW_STATUS_ERROR;
-    S->error = PCW_ERR_STATE;
-    return;
-  }
-  if (!S->controlled) return;
+  const int badbits = atomicAdd(&S->bad, 0);
+  S->last_bad = badbits;
+  S->bad = 0;
+  if (!S->controlled) return;  // host control: the host reads last_smax_* and last_bad
   pcw_step_record rec;
-  const int ok = tc_attempt(S->tc, sx, sy, I.dxi, I.deta, &rec);
+  int ok;
+  if (finish_checks(S, sx, sy, I.dxi, I.deta, badbits, &rec, &ok)) return;
   S->cfl_last = rec.cfl;
   if (ok) {
     S->cur = cur ^ 1;

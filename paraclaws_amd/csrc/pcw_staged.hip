@@ -75,7 +75,7 @@ __global__ void k_st_waves(StagedArgs a, int dir) {
   const int anybad = __any(bad);
   if ((threadIdx.x & 63) == 0 && (threadIdx.y * blockDim.x + threadIdx.x) % 64 == 0) {
     atomicMax(dir == 1 ? &S->smax_x : &S->smax_y, (unsigned long long)__double_as_longlong(m));
-    if (anybad) atomicOr(&S->bad, 1);
+    if (anybad) atomicOr(&S->bad, dir == 1 ? 1 : 2);
   }
 }
 
@@ -315,14 +315,13 @@ __global__ void k_st_finish(const KInst* inst, InstState* st, int n) {
   S->last_smax_x = sx;
   S->last_smax_y = sy;
   S->last_ok = 1;
-  if (S->bad) {
-    S->status = PCW_STATUS_ERROR;
-    S->error = PCW_ERR_STATE;
-    return;
-  }
+  const int badbits = S->bad;
+  S->last_bad = badbits;
+  S->bad = 0;
   if (!S->controlled) return;
   pcw_step_record rec;
-  const int ok = tc_attempt(S->tc, sx, sy, I.dxi, I.deta, &rec);
+  int ok;
+  if (finish_checks(S, sx, sy, I.dxi, I.deta, badbits, &rec, &ok)) return;
   S->cfl_last = rec.cfl;
   S->last_ok = ok;
   if (ok) S->accepted += 1;

@@ -17,6 +17,7 @@ contains
         use MOD_domain, only: solution_fields, mesh_fields
         use MOD_work_arrays, only: work_arrays
         use MOD_device, only: device_step
+        use MOD_error_handling, only: abort_program
         type(decomp_info), intent(in) :: decomposition
         type(time_control), intent(inout) :: t_ctrl
         type(work_arrays), intent(inout) :: work
@@ -24,15 +25,17 @@ contains
         type(solution_fields) :: solution
         integer, intent(inout) :: ierr
         double precision :: smax_xi, smax_eta
-        logical :: step_valid
+        logical :: step_valid, eta_failed
 
         ierr = -1
-        call device_step(t_ctrl%get_dt(), smax_xi, smax_eta)
+        call device_step(t_ctrl%get_dt(), smax_xi, smax_eta, eta_failed)
 
         ! xi sweep (MOD_solve.f90:71-120)
         call t_ctrl%update_max_CFL(smax_xi*t_ctrl%get_dt()/mesh%computational_space%d_xi)
         step_valid = t_ctrl%check_timestep_validity()
         if (.not. step_valid) return
+        ! the eta sweep runs only now in the reference: its NaN/complex check is fatal
+        if (eta_failed) call abort_program("NaN in the Jacobian or complex eigenvalues (eta sweep)")
 
         ! eta sweep (MOD_solve.f90:124-174)
         call t_ctrl%update_max_CFL(smax_eta*t_ctrl%get_dt()/mesh%computational_space%d_eta)

@@ -75,13 +75,19 @@ contains
         call device_check(pcw_fill_ghosts(handle), "pcw_fill_ghosts")
     end subroutine
 
-    subroutine device_step(dt, smax_xi, smax_eta)
-        !! one attempt of both sweeps (+ source + ghost fill) into the pending state
+    subroutine device_step(dt, smax_xi, smax_eta, eta_failed)
+        !! one attempt of both sweeps (+ source + ghost fill) into the pending
+        !! state; eta_failed: the eta sweep met a NaN Jacobian or complex
+        !! eigenvalues (fatal only if the xi sweep alone accepts the step)
         double precision, intent(in) :: dt
         double precision, intent(out) :: smax_xi, smax_eta
+        logical, intent(out) :: eta_failed
         real(c_double) :: dts(1), smax(2)
+        integer(c_int) :: rc
         dts(1) = dt
-        call device_check(pcw_step(handle, dts, smax), "pcw_step")
+        rc = pcw_step(handle, dts, smax)
+        eta_failed = rc == PCW_ERR_STATE_ETA
+        if (.not. eta_failed) call device_check(rc, "pcw_step")
         smax_xi = smax(1)
         smax_eta = smax(2)
     end subroutine

@@ -7,7 +7,7 @@ module MOD_pcw
     use iso_c_binding
     implicit none
 
-    integer(c_int), parameter :: PCW_OK = 0, PCW_REJECTED = 1
+    integer(c_int), parameter :: PCW_OK = 0, PCW_REJECTED = 1, PCW_ERR_STATE_ETA = -6
     integer(c_int), parameter :: PCW_FP64 = 0, PCW_FP32 = 1
     integer(c_int), parameter :: PCW_ENGINE_FUSED = 0, PCW_ENGINE_STAGED = 1
 

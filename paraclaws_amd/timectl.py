@@ -101,10 +101,21 @@ def solve_dimensional_split(engine, tc: TimeControl, dxi: float, deta: float) ->
     CFL bookkeeping and validity tests in the reference's order.  Returns
     1 (accepted) or -1 (rejected); the engine's pending state is kept or
     dropped by the caller."""
-    smax = engine.step(tc.dt)[0]
+    from paraclaws_amd import abi
+
+    eta_failed = False
+    try:
+        smax = engine.step(tc.dt)[0]
+    except abi.PcwError as e:
+        if e.code != abi.PCW_ERR_STATE_ETA:
+            raise
+        eta_failed = True  # fatal only if the xi sweep alone accepts the step
+        smax = engine.last_smax()[0]
     tc.update_max_CFL(smax[0] * tc.dt / dxi)
     if not tc.check_timestep_validity():
         return -1
+    if eta_failed:
+        raise abi.PcwError(abi.PCW_ERR_STATE, "NaN in the Jacobian or complex eigenvalues")
     tc.update_max_CFL(smax[1] * tc.dt / deta)
     return 1 if tc.check_timestep_validity() else -1
 

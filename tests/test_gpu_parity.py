@@ -181,6 +181,34 @@ def test_fp64_host_controlled_trajectory_with_rejection(oracle):
     assert np.array_equal(got[sides_mask(n, n)], q[sides_mask(n, n)])
 
 
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_fp64_device_controlled_rejection(oracle, engine):
+    """Same forced rejection with the device controller: the eta sweep of the
+    rejected attempt meets negative depths, which the reference never
+    computes (it rejects after the xi sweep), so it must not stop the run."""
+    abi = _abi()
+    O = oracle
+    n = 48
+    cfg = O.Config(nx=n, ny=n, limiter=2, bc=(2, 2, 2, 2))
+    ts = O.TimeSettings(t_final=0.05, dt0=0.05, cfl_target=0.9, dt_max=1e-2, n_snap=5)
+    q = O.dam_break(cfg)
+    q_init = q.copy()
+    recs, _ = O.run(cfg, ts, q)
+    _say(f"fp64 {engine} device-controlled run with rejection")
+    with abi.Engine([abi.Domain(n, n, bc=cfg.bc)], precision=abi.FP64,
+                    engine=abi.FUSED if engine == "fused" else abi.STAGED, limiter=2) as e:
+        e.set_state(0, q_init)
+        e.fill_ghosts()
+        e.time_init(t_final=ts.t_final, dt0=ts.dt0, cfl_target=ts.cfl_target, dt_max=ts.dt_max,
+                    dt_min=ts.dt_min, growth=ts.growth, shrink=ts.shrink, n_snapshots=ts.n_snap)
+        e.enable_log(len(recs) + 8)
+        e.run(steps_per_graph=8)
+        log = e.read_log(0)
+        got = e.get_state(0)
+    assert log == recs
+    assert np.array_equal(got[sides_mask(n, n)], q[sides_mask(n, n)])
+
+
 def test_batch_equals_solo_and_oracle(oracle):
     """Instances of uneven sizes in one handle: each equals the same instance
     run alone (bit for bit, FP32 and FP64), and the FP64 batch equals the oracle."""
