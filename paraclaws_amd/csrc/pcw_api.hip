@@ -138,14 +138,24 @@ int enqueue_fused(pcw_handle* h) {
   const int threads = 64 * h->wpb;
   int blocks = (h->ntiles + h->wpb - 1) / h->wpb;
   blocks = (int)round_up(blocks, 8);
+  // FP32: instances with and without source terms run in separate launches
+  const bool wetdry = h->p.dry_tol > 0.0;
+  bool any_src = wetdry, any_nosrc = !wetdry;
+  for (const auto& d : h->dom) {
+    any_src = any_src || d.source != PCW_SRC_NONE;
+    any_nosrc = any_nosrc || (d.source == PCW_SRC_NONE && !wetdry);
+  }
   for (int met = 0; met <= 1; ++met) {
     if (met ? !h->any_met : !h->any_plain) continue;
-    hipError_t e;
+    hipError_t e = hipSuccess;
     if (h->p.precision == PCW_FP32) {
       FusedArgs<float> a{(float*)h->pool, (const float*)h->bpool, (const float*)h->mpool,
                          h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
-                         h->p.dry_tol > 0.0, 0, h->p.grav, h->p.eps, h->p.dry_tol};
-      e = launch_fused_f32(limiter_code(h->p), met, a, blocks, threads, h->stream);
+                         wetdry, 0, h->p.grav, h->p.eps, h->p.dry_tol};
+      for (int phys = 0; phys <= 1 && e == hipSuccess; ++phys) {
+        if (phys ? !any_src : !any_nosrc) continue;
+        e = launch_fused_f32(limiter_code(h->p), met, phys, a, blocks, threads, h->stream);
+      }
     } else {
       FusedArgs<double> a{(double*)h->pool, (const double*)h->bpool, (const double*)h->mpool,
                           h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
@@ -311,6 +321,19 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
     I.xi_min = d.xi_min;
     I.eta_min = d.eta_min;
     I.manning = d.manning_n;
+    {
+      auto c = [](double lo, double dd, int i) { return lo + (double)(i - 1) * dd + dd / 2; };
+      for (int kk = 1; kk <= 2; ++kk) {
+        const double xl = c(I.xi_min, I.dxi, 1), xl2 = c(I.xi_min, I.dxi, 2);
+        const double xr = c(I.xi_min, I.dxi, d.nx), xr2 = c(I.xi_min, I.dxi, d.nx - 1);
+        const double yb = c(I.eta_min, I.deta, 1), yb2 = c(I.eta_min, I.deta, 2);
+        const double yt = c(I.eta_min, I.deta, d.ny), yt2 = c(I.eta_min, I.deta, d.ny - 1);
+        I.gr[kk - 1] = (float)((c(I.xi_min, I.dxi, 1 - kk) - xl) / (xl2 - xl));
+        I.gr[1 + kk] = (float)((c(I.xi_min, I.dxi, d.nx + kk) - xr) / (xr2 - xr));
+        I.gr[3 + kk] = (float)((c(I.eta_min, I.deta, 1 - kk) - yb) / (yb2 - yb));
+        I.gr[5 + kk] = (float)((c(I.eta_min, I.deta, d.ny + kk) - yt) / (yt2 - yt));
+      }
+    }
     if (d.source == PCW_SRC_BATHY || d.source == PCW_SRC_BATHY_FRICTION) {
       I.boff = bbase;
       bbase += I.plane;

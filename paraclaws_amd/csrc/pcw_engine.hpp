@@ -24,6 +24,10 @@ struct KInst {
   long long moff;      // metric planes (kappa, gamma_xi, gamma_eta) offset, -1: identity map
   double dxi, deta, xi_min, eta_min;
   double manning;
+  // FP32 extrapolation ratios (x_ghost - x_near) / (x_far - x_near) of the
+  // linear-extrapolation boundary, per side (left, right, bottom, top) and
+  // ghost layer (1, 2), from the reference's cell centres
+  float gr[8];
 };
 
 // time_control fields (MOD_time_control.f90:16-49), FP64.

@@ -21,6 +21,7 @@
 #include "pcw_device.hpp"
 #include "pcw_engine.hpp"
 #include "pcw_kernels.hpp"
+#include "pcw_tile.hpp"
 
 namespace pcw {
 
@@ -398,48 +399,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   // ------------------------------------------------ reductions + controller
   const T mx = wave_max(smx), my = wave_max(smy);
   const int bad_x = __any(bad & 1), bad_y = __any(bad & 2);
-  if (lane == 0) {
-    atomicMax(&S->smax_x, (unsigned long long)__double_as_longlong((double)mx));
-    atomicMax(&S->smax_y, (unsigned long long)__double_as_longlong((double)my));
-    if (bad_x | bad_y) atomicOr(&S->bad, (bad_x ? 1 : 0) | (bad_y ? 2 : 0));
-  }
-  __threadfence();
-  unsigned int prev = 0;
-  if (lane == 0) prev = atomicAdd(&S->tiles_done, 1u);
-  prev = __shfl(prev, 0, 64);
-  if (prev + 1 != (unsigned)I.ntiles) return;
-  // last tile of this instance: finish the attempt
-  __threadfence();
-  if (lane != 0) return;
-  const double sx = __longlong_as_double((long long)atomicAdd(&S->smax_x, 0ull));
-  const double sy = __longlong_as_double((long long)atomicAdd(&S->smax_y, 0ull));
-  S->smax_x = 0ull;
-  S->smax_y = 0ull;
-  S->last_smax_x = sx;
-  S->last_smax_y = sy;
-  S->tiles_done = 0u;
-  const int badbits = atomicAdd(&S->bad, 0);
-  S->last_bad = badbits;
-  S->bad = 0;
-  if (!S->controlled) return;  // host control: the host reads last_smax_* and last_bad
-  pcw_step_record rec;
-  int ok;
-  if (finish_checks(S, sx, sy, I.dxi, I.deta, badbits, &rec, &ok)) return;
-  S->cfl_last = rec.cfl;
-  if (ok) {
-    S->cur = cur ^ 1;
-    S->accepted += 1;
-  } else {
-    S->rejected += 1;
-  }
-  if (S->log && S->log_n < S->log_cap) S->log[S->log_n++] = rec;
-  if (S->tc.ended) {
-    S->status = PCW_STATUS_ERROR;
-    S->error = PCW_ERR_CONTROL;
-  } else if (!(S->tc.time < S->tc.time_final) ||
-             (S->max_accepted > 0 && S->accepted >= S->max_accepted)) {
-    S->status = PCW_STATUS_DONE;
-  }
+  tile_finish(S, I, cur, (double)mx, (double)my, bad_x != 0, bad_y != 0, lane);
 }
 
 // ------------------------------------------------------------- launcher
@@ -456,11 +416,6 @@ static hipError_t launch_lim(int lim, const FusedArgs<T>& a, int blocks, int thr
   return hipGetLastError();
 }
 
-hipError_t launch_fused_f32(int lim, bool met, const FusedArgs<float>& a, int blocks, int threads,
-                            hipStream_t s) {
-  return met ? launch_lim<float, false, true>(lim, a, blocks, threads, s)
-             : launch_lim<float, false, false>(lim, a, blocks, threads, s);
-}
 hipError_t launch_fused_f64(int lim, bool met, const FusedArgs<double>& a, int blocks, int threads,
                             hipStream_t s) {
   return met ? launch_lim<double, true, true>(lim, a, blocks, threads, s)

@@ -1,0 +1,580 @@
+// pcw_fused_f32.hip -- the FP32 fused step: one kernel per time step (xi
+// sweep, eta sweep, source term, physical ghost fill) for any number of
+// instances, built for VALU throughput on gfx950.
+//
+// Mapping (wave64, no LDS): as in the FP64 fused kernel, a wave owns a strip
+// of 60 output columns plus two halo lanes on each side and marches down the
+// rows of its tile; neighbour states in xi come from DPP lane shifts, and the
+// eta sweep runs in a register window two rows behind the xi sweep.
+//
+// What differs from the FP64 kernel:
+//   * Fluctuations are combined per interface: G = A+dQ - F goes to the
+//     cell on the interface's high side and H = A-dQ + F to the cell on its
+//     low side, so a cell update is q - dt/d * (G_i + H_{i+1}) and xi needs
+//     one lane shift of H instead of shifts of s2, w1 and F.
+//   * The eta window's per-row values live in three rotating slots (row r,
+//     r-1, r-2); the row loop is unrolled by three so that the rotation is a
+//     renaming and costs no register moves.  Input rows are prefetched three
+//     rows ahead into three buffers that rotate with the slots.
+//   * Rows whose every condition holds (interior rows of interior strips,
+//     away from the tile's first rows and the domain's top and bottom rows)
+//     run a body without any of the range tests; the other rows, and the
+//     strips on the domain's xi boundaries, run the general body.
+//   * Divisions are reciprocals, square roots are v_sqrt_f32 (1 ulp).
+//   * A cell with h <= 0 (or NaN) in a sweep's range is reported like the
+//     reference's NaN-in-Jacobian / complex-eigenvalue stop
+//     (SMOD_jakobian.f90:55-58, SMOD_eigenvalue_problem.f90:258-268), unless
+//     the wet/dry cut-off is enabled.
+//
+// Reference stages fused here (src/solver/MOD_solve.f90:23-175 and callees):
+// wave_decomposition/HLL (SMOD_approximate_Riemann_solvers.f90:189-224),
+// split_flux, integrate_flux_difference, the limiter
+// (MOD_flux_correction.f90:54-123), second_order_flux_correction,
+// integrator_flux_correction_increment, the CFL maximum (as the maximum wave
+// speed), integrate_source_term and apply_physical_boundaries.
+#include "pcw_device.hpp"
+#include "pcw_engine.hpp"
+#include "pcw_kernels.hpp"
+#include "pcw_tile.hpp"
+
+namespace pcw {
+namespace f32 {
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float vsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Buffer descriptors over one field plane of a tile's frame rows y0-1 ..
+// y1+3: row offsets go to the scalar offset, the lane's column to the 32-bit
+// vector offset.  A lane whose vector offset is beyond the descriptor's size
+// has its store dropped by the hardware range check, which masks stores
+// without a branch.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr unsigned kOff = 0x80000000u;  // out-of-range vector offset
+__device__ __forceinline__ rsrc_t make_rsrc(const float* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, bytes, 0x00020000);
+}
+__device__ __forceinline__ float bload(rsrc_t r, unsigned voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
+}
+__device__ __forceinline__ void bstore(float v, rsrc_t r, unsigned voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, soff, 0);
+}
+
+// One row's cell values and the interface below it, carried by the eta window.
+struct Slot {
+  float q[3];              // xi-updated state of the row
+  float s, f1, f2;         // eta speed |v| + c and flux components 1, 2 (component 0 is q[2])
+  float kap, fac;          // capacity and dt/deta/capacity of the row (metric arrays)
+  float gy;                // gamma_eta of the interface below the row (metric arrays)
+  float s2, w1[3], w2[3];  // unlimited waves of the interface below the row
+};
+
+// A prefetched input row.
+struct Pre {
+  float h, hu, hv;
+  float k, gx, gy;  // metric arrays
+};
+
+struct Ctx {
+  rsrc_t qi[3], qo[3];  // input and output planes h, hu, hv
+  rsrc_t mi[3];         // metric planes kappa, gamma_xi, gamma_eta (metric arrays)
+  rsrc_t bi;            // bathymetry
+  bool has_b;
+  int p4;               // row pitch in bytes
+  int rbase;            // frame row at the descriptors' origin (y0 - 1)
+  unsigned lo;          // load offset: this lane's frame column (clamped) * 4
+  unsigned so;          // store offset: frame column * 4, out of range off the output lanes
+  int lane, i, nx, ny, y0, y1;
+  int src, wetdry, x0, l_nx;
+  int bc[4];
+  float gr[8];       // extrapolation ratios of the linear-extrapolation boundary
+  float dtdx, dtdy, eps, g, dry, n2, gdx, gdy, dt;
+  bool out_lane, xface, xlim, check_cell;
+};
+
+__device__ __forceinline__ float readlane(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+// ghost value of one field from the first (near) and second (far) interior
+// cells next to a boundary: zero gradient (2), linear extrapolation (3) or
+// reflective wall (4, the normal momentum negated); layer kk = 1 or 2
+__device__ __forceinline__ float ghost(int bc, float near, float far, int kk, bool normal,
+                                       float ratio) {
+  if (bc == 2) return near;
+  if (bc == 3) return fmaf(far - near, ratio, near);
+  const float v = kk == 1 ? near : far;
+  return normal ? -v : v;
+}
+
+struct Red {
+  float smx, smy;
+  unsigned long long badx, bady;  // ballots of lanes that met h <= 0
+};
+
+// eta-direction (DIR 2) or xi-direction (DIR 1) speed and flux of one cell
+template <int DIR, bool WET>
+__device__ __forceinline__ void cell(float h, float hu, float hv, float g, float dry, float& s,
+                                     float& f1, float& f2) {
+  const float rh = rcp(h);
+  float u = hu * rh, v = hv * rh;
+  const float gh = g * h;
+  float gc = gh;
+  if (WET) {
+    const bool d = !(h > dry);
+    u = d ? 0.0f : u;
+    v = d ? 0.0f : v;
+    gc = gh > 0.0f ? gh : 0.0f;
+  }
+  const float c = vsqrt(gc);
+  if (DIR == 1) {
+    s = fabsf(u) + c;
+    f1 = fmaf(hu, u, gh * h);
+    f2 = hu * v;
+  } else {
+    s = fabsf(v) + c;
+    f1 = hv * u;
+    f2 = fmaf(hv, v, gh * h);
+  }
+}
+
+// HLL with the reference's symmetric speeds s1 = -s2 and regularised
+// denominator eps + s1 - s2; waves are zero when s2 <= eps.  The zeroing is
+// a multiplication by 0/1 (finite waves): a select here is turned into lane
+// branches by the compiler.
+__device__ __forceinline__ float hll(const float qL[3], const float fL[3], float sL,
+                                     const float qR[3], const float fR[3], float sR, float eps,
+                                     float w1[3], float w2[3]) {
+  const float s2 = fmaxf(sL, sR);
+  const float live = s2 > eps ? 1.0f : 0.0f;
+  const float rden = rcp(fmaf(-2.0f, s2, eps));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float t = fmaf(-s2, qR[k] + qL[k], fR[k] - fL[k]);
+    const float qs = t * rden;
+    w1[k] = (qs - qL[k]) * live;
+    w2[k] = (qR[k] - qs) * live;
+  }
+  return s2 * live;
+}
+
+// correction flux of one interface: 0.5 |s| (1 - |s| dt/d / kbar) (p1 w1 + p2 w2)
+__device__ __forceinline__ void corr_flux(float s2, float cfl, float p1, float p2,
+                                          const float w1[3], const float w2[3], float F[3]) {
+  const float h2 = 0.5f * s2;
+  const float coef = fmaf(-h2, cfl, h2);
+  const float c1 = coef * p1, c2 = coef * p2;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) F[k] = fmaf(c1, w1[k], c2 * w2[k]);
+}
+
+// source term of one finished cell (wet/dry cut-off, bathymetry slope,
+// semi-implicit Manning friction), FP32 with reciprocals
+__device__ __forceinline__ void source(float& h, float& hu, float& hv, int kind, float dry,
+                                       float g, float gdx, float gdy, float dt, float n2, float bx,
+                                       float by) {
+  if (!(h > dry)) {
+    h = h > 0.0f ? h : 0.0f;
+    hu = 0.0f;
+    hv = 0.0f;
+    return;
+  }
+  float nhu = hu, nhv = hv;
+  if (kind == PCW_SRC_BATHY || kind == PCW_SRC_BATHY_FRICTION) {
+    nhu = fmaf(dt, -gdx * h * bx, hu);
+    nhv = fmaf(dt, -gdy * h * by, hv);
+  }
+  if (kind == PCW_SRC_FRICTION || kind == PCW_SRC_BATHY_FRICTION) {
+    const float rh = rcp(h);
+    const float u = hu * rh, v = hv * rh;
+    const float sp = vsqrt(fmaf(u, u, v * v));
+    const float h43 = __builtin_amdgcn_exp2f(1.33333333f * __builtin_amdgcn_logf(h));
+    const float rden = rcp(fmaf(dt * g * n2, sp * rcp(h43), 1.0f));
+    nhu = nhu * rden;
+    nhv = nhv * rden;
+  }
+  hu = nhu;
+  hv = nhv;
+}
+
+template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
+__device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const Slot& A,
+                                       const Slot& B, float acc[3], float bm, float b0,
+                                       float bnew, float fin1[3], float finm[3]);
+
+// One row r of a wave's march.  N: slot of row r (written), A: row r-1, B:
+// row r-2.  P: this row's prefetched input, reloaded with row r+3.
+// GEN: the general body with every range test; otherwise the row must be
+// an interior row of an interior strip (see the kernel's chunk test).
+template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
+__device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot& B, Pre& P,
+                                    float acc[3], Red& R, float& bm, float& b0, float fin1[3],
+                                    float finm[3]) {
+  // ---------------------------------------------------------- input row
+  float q[3] = {P.h, P.hu, P.hv};
+  const float mk = P.k, mgx = P.gx, mgy = P.gy;
+  float bnew = 0.0f;
+  if (PHYS && C.has_b && r >= C.y0) bnew = bload(C.bi, C.lo, (r - C.rbase) * C.p4);  // b(i, r-1)
+  const bool wet = PHYS && C.wetdry;
+
+  // ---------------------------------------------------------- xi sweep
+  float qx[3] = {q[0], q[1], q[2]};
+  if (!GEN || (r >= 1 && r <= C.ny)) {
+    float s, f1, f2;
+    if (wet) cell<1, true>(q[0], q[1], q[2], C.g, C.dry, s, f1, f2);
+    else cell<1, false>(q[0], q[1], q[2], C.g, C.dry, s, f1, f2);
+    if (!wet) {
+      const unsigned long long b = __ballot(!(q[0] > 0.0f));
+      R.badx |= GEN ? (b & __ballot(C.check_cell)) : b;
+    }
+    const float qL[3] = {shr1(q[0]), shr1(q[1]), shr1(q[2])};
+    const float fL[3] = {qL[1], shr1(f1), shr1(f2)};
+    const float fR[3] = {q[1], f1, f2};
+    const float sL = shr1(s);
+    float w1[3], w2[3];
+    const float s2 = hll(qL, fL, sL, q, fR, s, C.eps, w1, w2);
+    R.smx = fmaxf(R.smx, (!GEN || C.xface) ? s2 : 0.0f);
+    float F[3] = {0.0f, 0.0f, 0.0f};
+    const float kap_l = MET ? shr1(mk) : 1.0f;
+    if (HO) {
+      float p1 = 1.0f, p2 = 1.0f;
+      if (LIM > 0) {
+        const float w1R[3] = {shl1(w1[0]), shl1(w1[1]), shl1(w1[2])};
+        const float w2L[3] = {shr1(w2[0]), shr1(w2[1]), shr1(w2[2])};
+        p1 = limit_fast<LIM>(w1, w1R, C.eps);
+        p2 = limit_fast<LIM>(w2, w2L, C.eps);
+        if (GEN) {
+          p1 = C.xlim ? p1 : 1.0f;
+          p2 = C.xlim ? p2 : 1.0f;
+        }
+      }
+      float cfl = s2 * C.dtdx;
+      if (MET) cfl = cfl * rcp(0.5f * (kap_l + mk));
+      corr_flux(s2, cfl, p1, p2, w1, w2, F);
+    }
+    const float sg = MET ? mgx * s2 : s2;
+    float G[3], HR[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      G[k] = fmaf(sg, w2[k], -F[k]);
+      HR[k] = shl1(fmaf(-sg, w1[k], F[k]));
+    }
+    const float fac = MET ? C.dtdx * rcp(mk) : C.dtdx;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) qx[k] = fmaf(-fac, G[k] + HR[k], q[k]);
+  }
+
+  // ---------------------------------------------------------- eta sweep
+#pragma unroll
+  for (int k = 0; k < 3; ++k) N.q[k] = qx[k];
+  if (wet) cell<2, true>(qx[0], qx[1], qx[2], C.g, C.dry, N.s, N.f1, N.f2);
+  else cell<2, false>(qx[0], qx[1], qx[2], C.g, C.dry, N.s, N.f1, N.f2);
+  if (!wet && (!GEN || r <= C.ny + 1)) R.bady |= __ballot(!(qx[0] > 0.0f) && C.out_lane);
+  if (MET) {
+    N.kap = mk;
+    N.fac = C.dtdy * rcp(mk);
+    N.gy = mgy;
+  }
+  // interface r (rows r-1 | r)
+  if (!GEN || r >= C.y0 - 1) {
+    const float fA[3] = {A.q[2], A.f1, A.f2}, fN[3] = {qx[2], N.f1, N.f2};
+    N.s2 = hll(A.q, fA, A.s, qx, fN, N.s, C.eps, N.w1, N.w2);
+    const bool in = C.out_lane && (!GEN || (r >= 0 && r <= C.ny + 1));
+    R.smy = fmaxf(R.smy, in ? N.s2 : 0.0f);
+  } else {
+    N.s2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) N.w1[k] = N.w2[k] = 0.0f;
+  }
+  // interface r-1: limiter, correction flux, G and H
+  if (!GEN || r >= C.y0 + 1) finish<LIM, HO, MET, PHYS, GEN>(C, r, N, A, B, acc, bm, b0, bnew, fin1, finm);
+  if (PHYS) {
+    bm = b0;
+    b0 = bnew;
+  }
+  // prefetch row r+3 (clamped to the tile's last row) into this row's
+  // buffer, after the row's last use of it so that the load can write the
+  // same registers
+  {
+    const int so = (min(r + 3, C.y1 + 2) + 1 - C.rbase) * C.p4;
+    P.h = bload(C.qi[0], C.lo, so);
+    P.hu = bload(C.qi[1], C.lo, so);
+    P.hv = bload(C.qi[2], C.lo, so);
+    if (MET) {
+      P.k = bload(C.mi[0], C.lo, so);
+      P.gx = bload(C.mi[1], C.lo, so);
+      P.gy = bload(C.mi[2], C.lo, so);
+    }
+  }
+}
+
+// The part of a row that needs interface r-1 (from row y0+1 on): its
+// limiter and correction flux, row r-2's last contribution, source term,
+// stores and ghosts, and row r-1's first contribution.
+template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
+__device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const Slot& A,
+                                       const Slot& B, float acc[3], float bm, float b0,
+                                       float bnew, float fin1[3], float finm[3]) {
+  float Fa[3] = {0.0f, 0.0f, 0.0f};
+  if (HO) {
+    float p1 = 1.0f, p2 = 1.0f;
+    if (LIM > 0 && (!GEN || (r - 1 >= 1 && r - 1 <= C.ny))) {
+      p1 = limit_fast<LIM>(A.w1, N.w1, C.eps);
+      p2 = limit_fast<LIM>(A.w2, B.w2, C.eps);
+    }
+    float cfl = A.s2 * C.dtdy;
+    if (MET) cfl = cfl * rcp(0.5f * (B.kap + A.kap));
+    corr_flux(A.s2, cfl, p1, p2, A.w1, A.w2, Fa);
+  }
+  const float sg = MET ? A.gy * A.s2 : A.s2;
+  float G[3], H[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    G[k] = fmaf(sg, A.w2[k], -Fa[k]);
+    H[k] = fmaf(-sg, A.w1[k], Fa[k]);
+  }
+  if (!GEN || r >= C.y0 + 2) {  // finish row j = r-2
+    const int j = r - 2;
+    const float facB = MET ? B.fac : C.dtdy;
+    float qn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) qn[k] = fmaf(-facB, H[k], acc[k]);
+    if (PHYS && (C.src != PCW_SRC_NONE || C.wetdry)) {
+      const float bx = shl1(b0) - shr1(b0), by = bnew - bm;
+      source(qn[0], qn[1], qn[2], C.src, C.dry, C.g, C.gdx, C.gdy, C.dt, C.n2, bx, by);
+    }
+    const int sj = (j + 1 - C.rbase) * C.p4;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) bstore(qn[k], C.qo[k], C.so, sj);
+    if (GEN) {
+      // xi-boundary ghosts of row j from this row's finished cells (near:
+      // column 1 or nx, far: column 2 or nx-1, read as wave-uniform values)
+      if (C.x0 == 1) {
+        float nr[3], fr[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { nr[k] = readlane(qn[k], 2); fr[k] = readlane(qn[k], 3); }
+        if (C.lane <= 1) {
+          const int kk = 2 - C.lane;  // lane 1: ghost column 0, lane 0: column -1
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            bstore(ghost(C.bc[0], nr[k], fr[k], kk, k == 1, kk == 1 ? C.gr[0] : C.gr[1]), C.qo[k],
+                   4u * C.lane, sj);
+        }
+      }
+      if (C.l_nx >= 3 && C.l_nx <= 61) {
+        float nr[3], fr[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { nr[k] = readlane(qn[k], C.l_nx); fr[k] = readlane(qn[k], C.l_nx - 1); }
+        const int kk = C.lane - C.l_nx;
+        if (kk == 1 || kk == 2) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            bstore(ghost(C.bc[1], nr[k], fr[k], kk, k == 1, kk == 1 ? C.gr[2] : C.gr[3]), C.qo[k],
+                   4u * (C.nx + 1 + kk), sj);
+        }
+      }
+      // eta-boundary ghosts
+      if (j == 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fin1[k] = qn[k];
+      }
+      if (j == 2 && C.y0 == 1) {
+#pragma unroll
+        for (int kk = 1; kk <= 2; ++kk) {
+          const int sg = (2 - kk - C.rbase) * C.p4;  // ghost row 1-kk
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            bstore(ghost(C.bc[2], fin1[k], qn[k], kk, k == 2, C.gr[3 + kk]), C.qo[k], C.so, sg);
+        }
+      }
+      if (j == C.ny) {
+#pragma unroll
+        for (int kk = 1; kk <= 2; ++kk) {
+          const int sg = (C.ny + kk + 1 - C.rbase) * C.p4;  // ghost row ny+kk
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            bstore(ghost(C.bc[3], qn[k], finm[k], kk, k == 2, C.gr[5 + kk]), C.qo[k], C.so, sg);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) finm[k] = qn[k];
+    }
+  }
+  // row r-1: xi-updated state minus the contribution of its lower interface
+  const float facA = MET ? A.fac : C.dtdy;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) acc[k] = fmaf(-facA, G[k], A.q[k]);
+}
+
+template <int LIM, bool HO, bool MET, bool PHYS>
+__global__ void __launch_bounds__(256) k_step(FusedArgs<float> a) {
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous
+  // range of tiles so neighbouring strips share that XCD's L2.
+  const int nb = gridDim.x, per = nb >> 3;
+  const int bb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int tid = __builtin_amdgcn_readfirstlane(bb * wpb + (threadIdx.x >> 6));
+  if (tid >= a.ntiles) return;
+  const int4 t = a.tiles[tid];
+  const int in = __builtin_amdgcn_readfirstlane(t.x);
+  const KInst& I = a.inst[in];
+  InstState* S = a.st + in;
+  if (S->status != PCW_STATUS_RUNNING) return;
+  if (MET != (I.moff >= 0)) return;  // instances with and without metric arrays run in separate launches
+  const bool phys_inst = I.source != PCW_SRC_NONE || a.wetdry;
+  if (PHYS != phys_inst) return;      // likewise instances with and without source terms
+
+  Ctx C;
+  const int cur = S->cur;
+  const double dt = S->tc.dt;
+  C.nx = I.nx;
+  C.ny = I.ny;
+  C.x0 = t.y;
+  C.y0 = t.z;
+  C.y1 = t.w;
+  C.lane = lane;
+  C.i = C.x0 - 2 + lane;                              // this lane's column (1-based interior)
+  C.l_nx = C.nx - C.x0 + 2;                           // lane of column nx
+  C.out_lane = lane >= 2 && lane <= 61 && C.i >= 1 && C.i <= C.nx;
+  C.p4 = I.pitch * 4;
+  C.rbase = C.y0 - 1;
+  C.lo = 4u * (unsigned)(C.i + 1 < C.nx + 3 ? C.i + 1 : C.nx + 3);
+  C.so = C.out_lane ? 4u * (unsigned)(C.i + 1) : kOff;
+  {
+    const long long o = (long long)C.rbase * I.pitch;  // descriptor origin within a plane
+    const int bytes = (C.y1 - C.y0 + 5) * C.p4;
+    const float* qin = a.pool + I.off[cur] + o;
+    const float* qout = a.pool + I.off[cur ^ 1] + o;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      C.qi[k] = make_rsrc(qin + k * I.plane, bytes);
+      C.qo[k] = make_rsrc(qout + k * I.plane, bytes);
+      C.mi[k] = MET ? make_rsrc(a.mpool + I.moff + o + k * I.plane, bytes) : C.qi[k];
+    }
+    const bool use_b = I.boff >= 0 && (I.source == PCW_SRC_BATHY || I.source == PCW_SRC_BATHY_FRICTION);
+    C.has_b = PHYS && use_b;
+    C.bi = C.has_b ? make_rsrc(a.bpool + I.boff + o, bytes) : C.qi[0];
+  }
+  C.xface = lane >= 1 && C.i >= 0 && C.i <= C.nx + 1;  // interface i-1/2 in the xi CFL range
+  C.xlim = C.i >= 1 && C.i <= C.nx;                   // interface limited
+  C.check_cell = C.i >= -1 && C.i <= C.nx + 1;
+  C.src = I.source;
+  C.wetdry = a.wetdry;
+  for (int k = 0; k < 4; ++k) C.bc[k] = I.bc[k];
+  for (int k = 0; k < 8; ++k) C.gr[k] = I.gr[k];
+  C.dtdx = (float)(dt / I.dxi);
+  C.dtdy = (float)(dt / I.deta);
+  C.eps = (float)a.eps;
+  C.g = (float)a.g;
+  C.dry = (float)a.dry;
+  C.n2 = (float)(I.manning * I.manning);
+  C.gdx = (float)(a.g / I.dxi);
+  C.gdy = (float)(a.g / I.deta);
+  C.dt = (float)dt;
+
+  Red R;
+  R.smx = R.smy = 0.0f;
+  R.badx = R.bady = 0ull;
+  Slot X, Y, Z;
+  X.s2 = Y.s2 = Z.s2 = 0.0f;
+  X.s = Y.s = Z.s = 0.0f;
+  X.f1 = Y.f1 = Z.f1 = X.f2 = Y.f2 = Z.f2 = 0.0f;
+  X.kap = Y.kap = Z.kap = X.fac = Y.fac = Z.fac = X.gy = Y.gy = Z.gy = 1.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    X.q[k] = Y.q[k] = Z.q[k] = 0.0f;
+    X.w1[k] = Y.w1[k] = Z.w1[k] = X.w2[k] = Y.w2[k] = Z.w2[k] = 0.0f;
+  }
+  float acc[3] = {0.0f, 0.0f, 0.0f};
+  float fin1[3] = {0.0f, 0.0f, 0.0f}, finm[3] = {0.0f, 0.0f, 0.0f};
+  float bm = 0.0f, b0 = 0.0f;
+
+  // prefetch rows y0-2, y0-1, y0
+  Pre P0, P1, P2;
+  {
+    Pre* ps[3] = {&P0, &P1, &P2};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int so = d * C.p4;  // frame rows y0-1 .. y0+1
+      ps[d]->h = bload(C.qi[0], C.lo, so);
+      ps[d]->hu = bload(C.qi[1], C.lo, so);
+      ps[d]->hv = bload(C.qi[2], C.lo, so);
+      ps[d]->k = ps[d]->gx = ps[d]->gy = 1.0f;
+      if (MET) {
+        ps[d]->k = bload(C.mi[0], C.lo, so);
+        ps[d]->gx = bload(C.mi[1], C.lo, so);
+        ps[d]->gy = bload(C.mi[2], C.lo, so);
+      }
+    }
+  }
+
+  // Rows y0-2 .. y1+2 in chunks of three.  A chunk runs the unconditional
+  // body when the strip is away from the xi boundaries and all three rows
+  // are past the tile's start-up rows and the domain's bottom ghost rows
+  // (rs) and before the domain's top rows (re); those chunks form one run.
+  // The run is its own loop, entered with no memory operation in flight, so
+  // that the wait for each prefetched row counts only the run's own loads
+  // and stores.
+  const bool edge = C.x0 == 1 || (C.l_nx >= 3 && C.l_nx <= 61);
+  const int rs = C.y0 + 2 + (C.y0 == 1 ? 2 : 0);
+  const int re = min(C.y1 + 2, C.ny);
+  const int rl = C.y1 + 2;
+  int r = C.y0 - 2;
+  while (r <= rl) {
+    if (edge || r < rs || r + 2 > re) {
+      row<LIM, HO, MET, PHYS, true>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
+      if (r + 1 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
+      if (r + 2 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 2, Y, Z, X, P2, acc, R, bm, b0, fin1, finm);
+      r += 3;
+      continue;
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    do {
+      row<LIM, HO, MET, PHYS, false>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 2, Y, Z, X, P2, acc, R, bm, b0, fin1, finm);
+      r += 3;
+    } while (r + 2 <= re);
+  }
+
+  const float mx = wave_max(R.smx), my = wave_max(R.smy);
+  tile_finish(S, I, cur, (double)mx, (double)my, R.badx != 0ull, R.bady != 0ull, lane);
+}
+
+template <bool HO, bool MET, bool PHYS>
+static hipError_t launch_lim(int lim, const FusedArgs<float>& a, int blocks, int threads,
+                             hipStream_t s) {
+  switch (lim) {
+    case 0: k_step<0, HO, MET, PHYS><<<blocks, threads, 0, s>>>(a); break;
+    case 1: k_step<1, HO, MET, PHYS><<<blocks, threads, 0, s>>>(a); break;
+    case 2: k_step<2, HO, MET, PHYS><<<blocks, threads, 0, s>>>(a); break;
+    case 3: k_step<3, HO, MET, PHYS><<<blocks, threads, 0, s>>>(a); break;
+    default: k_step<4, HO, MET, PHYS><<<blocks, threads, 0, s>>>(a); break;
+  }
+  return hipGetLastError();
+}
+
+template <bool MET, bool PHYS>
+static hipError_t launch_ho(int lim, const FusedArgs<float>& a, int blocks, int threads,
+                            hipStream_t s) {
+  if (!a.ho) {
+    k_step<0, false, MET, PHYS><<<blocks, threads, 0, s>>>(a);
+    return hipGetLastError();
+  }
+  return launch_lim<true, MET, PHYS>(lim, a, blocks, threads, s);
+}
+
+}  // namespace f32
+
+hipError_t launch_fused_f32(int lim, bool met, bool phys, const FusedArgs<float>& a, int blocks,
+                            int threads, hipStream_t s) {
+  if (met) {
+    return phys ? f32::launch_ho<true, true>(lim, a, blocks, threads, s)
+                : f32::launch_ho<true, false>(lim, a, blocks, threads, s);
+  }
+  return phys ? f32::launch_ho<false, true>(lim, a, blocks, threads, s)
+              : f32::launch_ho<false, false>(lim, a, blocks, threads, s);
+}
+
+}  // namespace pcw

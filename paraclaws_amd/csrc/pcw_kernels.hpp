@@ -46,7 +46,7 @@ struct FusedArgs {
   double g, eps, dry;
 };
 
-hipError_t launch_fused_f32(int lim, bool met, const FusedArgs<float>& a, int blocks,
+hipError_t launch_fused_f32(int lim, bool met, bool phys, const FusedArgs<float>& a, int blocks,
                             int threads, hipStream_t s);
 hipError_t launch_fused_f64(int lim, bool met, const FusedArgs<double>& a, int blocks,
                             int threads, hipStream_t s);

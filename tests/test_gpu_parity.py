@@ -77,6 +77,47 @@ def test_fp64_step_bitwise(oracle, engine, case):
     assert cfl == cfl_ref
 
 
+FP32_CASES = CASES + [
+    # grids with interior strips and long tiles, so that the FP32 kernel's
+    # branch-free runs of rows are used; the last field is tile_rows
+    (250, 301, 4, True, (4, 3, 2, 4), "rough", 64),
+    (247, 190, 2, True, (3, 2, 4, 3), "smooth", 16),
+    (181, 97, 1, True, (2, 4, 3, 2), "rough", 37),
+]
+
+
+@pytest.mark.parametrize("case", FP32_CASES, ids=[f"{c[0]}x{c[1]}-lim{c[2]}-ho{int(c[3])}-bc{''.join(map(str, c[4]))}-{c[5]}" + (f"-t{c[6]}" if len(c) > 6 else "") for c in FP32_CASES])
+def test_fp32_step_close_cases(oracle, case):
+    """One FP32 fused step against the FP64 oracle on every BC combination,
+    limiter and grid shape of the FP64 cases plus grids with interior strips:
+    interior and ghost cells within 4e-6 of the field scale, sweep speeds
+    within 1e-6 relative."""
+    abi = _abi()
+    O = oracle
+    nx, ny, lim, ho, bc, kind = case[:6]
+    tile_rows = case[6] if len(case) > 6 else 0
+    q0 = smooth_state(nx, ny, seed=nx + ny) if kind == "smooth" else rough_state(nx, ny, seed=nx * ny)
+    cfg = O.Config(nx=nx, ny=ny, limiter=max(lim, 1), use_limiter=lim > 0, higher_order=ho, bc=bc)
+    O.fill_ghosts(cfg, q0)
+    dt = stable_dt(q0, nx, ny)
+    ref = q0.copy()
+    cfl_ref = _oracle_step(O, cfg, ref, dt)
+    _say(f"fp32 fused step {case}")
+    with abi.Engine([abi.Domain(nx, ny, bc=bc)], precision=abi.FP32, higher_order=ho,
+                    use_limiter=lim > 0, limiter=max(lim, 1), tile_rows=tile_rows) as e:
+        e.set_state(0, q0)
+        smax = e.step(dt)[0]
+        e.accept()
+        got = e.get_state(0)
+    m = sides_mask(nx, ny)
+    for k in range(3):
+        d = np.abs(got[..., k][m] - ref[..., k][m])
+        assert d.max() <= 4e-6 * max(1.0, np.abs(ref[..., k][m]).max()), (k, d.max())
+    dxi, deta = 2.0 / nx, 2.0 / ny
+    cfl = max(smax[0] * dt / dxi, smax[1] * dt / deta)
+    assert abs(cfl - cfl_ref) <= 1e-6 * cfl_ref
+
+
 @pytest.mark.parametrize("ixy", [1, 2])
 def test_fp64_single_sweep_bitwise(oracle, ixy):
     abi = _abi()
@@ -263,9 +304,30 @@ def test_totals(oracle):
         assert np.allclose(tot, ref, rtol=1e-13, atol=1e-12), (prec, tot, ref)
 
 
-def test_error_on_dry_cell(oracle):
-    """h = 0 in the interior: the reference stops ("isnan in jakobi");
-    the FP64 engine reports PCW_ERR_STATE."""
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_error_on_dry_cell(oracle, prec):
+    """A dry interior cell (h = hu = hv = 0): the xi Jacobian holds 0/0, the
+    reference stops ("isnan in jakobi", SMOD_jakobian.f90:55-58) in the xi
+    sweep; both engines report PCW_ERR_STATE (FP32: h <= 0 in the sweep's
+    range)."""
+    abi = _abi()
+    nx = ny = 20
+    q = smooth_state(nx, ny, seed=1)
+    q[10, 10, :] = 0.0
+    precision = abi.FP64 if prec == "fp64" else abi.FP32
+    with abi.Engine([abi.Domain(nx, ny)], precision=precision) as e:
+        e.set_state(0, q)
+        with pytest.raises(abi.PcwError) as ei:
+            e.step(1e-4)
+        assert ei.value.code == abi.PCW_ERR_STATE
+
+
+def test_fp64_zero_depth_with_momentum_reports_eta():
+    """h = 0 with non-zero momenta: the xi Jacobian has infinite entries but
+    no NaN, so the xi sweep itself does not stop; its speed is infinite (the
+    step would be rejected) and the eta sweep then meets NaN.  pcw_step
+    reports PCW_ERR_STATE_ETA and leaves the xi speed for the caller's
+    validity test."""
     abi = _abi()
     nx = ny = 20
     q = smooth_state(nx, ny, seed=1)
@@ -274,7 +336,8 @@ def test_error_on_dry_cell(oracle):
         e.set_state(0, q)
         with pytest.raises(abi.PcwError) as ei:
             e.step(1e-4)
-        assert ei.value.code == abi.PCW_ERR_STATE
+        assert ei.value.code == abi.PCW_ERR_STATE_ETA
+        assert np.isinf(e.last_smax()[0][0])
 
 
 def test_user_bc_rejected():
