@@ -770,33 +770,69 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
   }
   rc = push_states(h);
   if (rc) return rc;
-  const int rows_c[] = {16, 32, 64, 128, 256};
+  // Candidates: powers of two, and for FP32 the tile heights whose tile
+  // count just fits k rounds of the resident waves (k = 1..6), so that the
+  // last round is full.
   const int wpb_c[] = {1, 2, 4};
+  std::vector<std::pair<int, int>> cand;
+  int cu = 0;
+  (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, 0);
+  bool any_src = h->p.dry_tol > 0.0, any_nosrc = !(h->p.dry_tol > 0.0);
+  for (const auto& d : h->dom) {
+    any_src = any_src || d.source != PCW_SRC_NONE;
+    any_nosrc = any_nosrc || (d.source == PCW_SRC_NONE && !(h->p.dry_tol > 0.0));
+  }
+  for (int wv : wpb_c) {
+    for (int tr : {16, 32, 64, 128, 256}) cand.push_back({tr, wv});
+    if (h->p.precision != PCW_FP32 || cu <= 0 || (any_src && any_nosrc) || (h->any_met && h->any_plain))
+      continue;
+    const int bpc = fused_f32_blocks_per_cu(limiter_code(h->p), h->p.higher_order != 0, h->any_met,
+                                            any_src, 64 * wv);
+    if (bpc <= 0) continue;
+    const long long slots = (long long)bpc * wv * cu;  // resident waves
+    auto ntiles = [&](int tr) {
+      long long t = 0;
+      for (const auto& d : h->dom)
+        t += (long long)((d.nx + kStripOut - 1) / kStripOut) * ((d.ny + tr - 1) / tr);
+      return t;
+    };
+    for (int k = 1; k <= 6; ++k) {
+      int lo = 8, hi = std::max(8, h->max_ny);
+      if (ntiles(hi) > k * slots) continue;
+      while (lo < hi) {  // smallest height whose tile count fits k rounds
+        const int mid = (lo + hi) / 2;
+        if (ntiles(mid) <= k * slots) hi = mid; else lo = mid + 1;
+      }
+      cand.push_back({lo, wv});
+    }
+  }
+  std::sort(cand.begin(), cand.end());
+  cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
   double best = 1e30;
   int br = h->tile_rows, bw = h->wpb;
-  for (int tr : rows_c)
-    for (int wv : wpb_c) {
-      h->wpb = wv;
-      build_tiles(h, tr);
-      rc = upload_tiles(h);
+  for (const auto& c : cand) {
+    const int tr = c.first, wv = c.second;
+    h->wpb = wv;
+    build_tiles(h, tr);
+    rc = upload_tiles(h);
+    if (rc) return rc;
+    rc = enqueue_fused(h);  // warm-up
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    for (int s = 0; s < steps; ++s) {
+      rc = enqueue_fused(h);
       if (rc) return rc;
-      rc = enqueue_fused(h);  // warm-up
-      if (rc) return rc;
-      HIPCHK(hipEventRecord(h->ev0, h->stream));
-      for (int s = 0; s < steps; ++s) {
-        rc = enqueue_fused(h);
-        if (rc) return rc;
-      }
-      HIPCHK(hipEventRecord(h->ev1, h->stream));
-      HIPCHK(hipEventSynchronize(h->ev1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-      if (ms < best) {
-        best = ms;
-        br = tr;
-        bw = wv;
-      }
     }
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (ms < best) {
+      best = ms;
+      br = tr;
+      bw = wv;
+    }
+  }
   h->tile_rows = br;
   h->wpb = bw;
   build_tiles(h, br);

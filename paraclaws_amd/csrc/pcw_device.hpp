@@ -20,12 +20,13 @@ namespace pcw {
 
 // --------------------------------------------------------------- lane shifts
 // DPP wave_shr:1 / wave_shl:1 (GFX9 family): lane l receives lane l-1 / l+1.
-// Lanes shifted in from outside the wave receive 0.
+// Lanes shifted in from outside the wave receive 0 (bound_ctrl), so every
+// lane is written and no old value has to be set up.
 __device__ __forceinline__ int shr1_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int shl1_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true);
 }
 __device__ __forceinline__ float shr1(float v) {
   return __builtin_bit_cast(float, shr1_i(__builtin_bit_cast(int, v)));

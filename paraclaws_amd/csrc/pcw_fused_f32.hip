@@ -43,6 +43,17 @@ namespace f32 {
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float vsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// max of two speeds (non-negative floats, or NaN) as integers: one
+// instruction, and a NaN operand wins (it is reported as an error at the end
+// of the tile) instead of being dropped as by fmaxf
+__device__ __forceinline__ float smax2(float a, float b) {
+  return __builtin_bit_cast(float, max(__builtin_bit_cast(int, a), __builtin_bit_cast(int, b)));
+}
+__device__ __forceinline__ float wave_smax(float v) {
+  for (int o = 32; o >= 1; o >>= 1) v = smax2(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
 // Buffer descriptors over one field plane of a tile's frame rows y0-1 ..
 // y1+3: row offsets go to the scalar offset, the lane's column to the 32-bit
 // vector offset.  A lane whose vector offset is beyond the descriptor's size
@@ -88,8 +99,9 @@ struct Ctx {
   int src, wetdry, x0, l_nx;
   int bc[4];
   float gr[8];       // extrapolation ratios of the linear-extrapolation boundary
-  float dtdx, dtdy, eps, g, dry, n2, gdx, gdy, dt;
-  bool out_lane, xface, xlim, check_cell;
+  float dtdx, dtdy, hdtdx, hdtdy, eps, g, dry, n2, gdx, gdy, dt;
+  bool out_lane, xface, xlim;
+  unsigned long long xmask, ymask;  // lanes in the xi sweep's cell range / the output lanes
 };
 
 __device__ __forceinline__ float readlane(float v, int l) {
@@ -139,30 +151,38 @@ __device__ __forceinline__ void cell(float h, float hu, float hv, float g, float
 }
 
 // HLL with the reference's symmetric speeds s1 = -s2 and regularised
-// denominator eps + s1 - s2; waves are zero when s2 <= eps.  The zeroing is
-// a multiplication by 0/1 (finite waves): a select here is turned into lane
-// branches by the compiler.
+// denominator eps + s1 - s2; waves are zero when s2 <= eps.  Such
+// interfaces (both cells at rest and dry to within eps) are rare, so the
+// zeroing runs only when a lane of the wave has one; it is a multiplication
+// by 0/1 because the compiler turns a select here into lane branches.
 __device__ __forceinline__ float hll(const float qL[3], const float fL[3], float sL,
                                      const float qR[3], const float fR[3], float sR, float eps,
                                      float w1[3], float w2[3]) {
-  const float s2 = fmaxf(sL, sR);
-  const float live = s2 > eps ? 1.0f : 0.0f;
+  float s2 = smax2(sL, sR);
   const float rden = rcp(fmaf(-2.0f, s2, eps));
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const float t = fmaf(-s2, qR[k] + qL[k], fR[k] - fL[k]);
     const float qs = t * rden;
-    w1[k] = (qs - qL[k]) * live;
-    w2[k] = (qR[k] - qs) * live;
+    w1[k] = qs - qL[k];
+    w2[k] = qR[k] - qs;
   }
-  return s2 * live;
+  if (__builtin_expect(__ballot(!(s2 > eps)) != 0ull, 0)) {
+    const float live = s2 > eps ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      w1[k] *= live;
+      w2[k] *= live;
+    }
+    s2 *= live;
+  }
+  return s2;
 }
 
-// correction flux of one interface: 0.5 |s| (1 - |s| dt/d / kbar) (p1 w1 + p2 w2)
-__device__ __forceinline__ void corr_flux(float s2, float cfl, float p1, float p2,
-                                          const float w1[3], const float w2[3], float F[3]) {
-  const float h2 = 0.5f * s2;
-  const float coef = fmaf(-h2, cfl, h2);
+// correction flux of one interface: coef (p1 w1 + p2 w2) with
+// coef = 0.5 |s| (1 - |s| dt/d / kbar)
+__device__ __forceinline__ void corr_flux(float coef, float p1, float p2, const float w1[3],
+                                          const float w2[3], float F[3]) {
   const float c1 = coef * p1, c2 = coef * p2;
 #pragma unroll
   for (int k = 0; k < 3; ++k) F[k] = fmaf(c1, w1[k], c2 * w2[k]);
@@ -225,7 +245,7 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
     else cell<1, false>(q[0], q[1], q[2], C.g, C.dry, s, f1, f2);
     if (!wet) {
       const unsigned long long b = __ballot(!(q[0] > 0.0f));
-      R.badx |= GEN ? (b & __ballot(C.check_cell)) : b;
+      R.badx |= GEN ? (b & C.xmask) : b;
     }
     const float qL[3] = {shr1(q[0]), shr1(q[1]), shr1(q[2])};
     const float fL[3] = {qL[1], shr1(f1), shr1(f2)};
@@ -233,7 +253,7 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
     const float sL = shr1(s);
     float w1[3], w2[3];
     const float s2 = hll(qL, fL, sL, q, fR, s, C.eps, w1, w2);
-    R.smx = fmaxf(R.smx, (!GEN || C.xface) ? s2 : 0.0f);
+    R.smx = smax2(R.smx, (!GEN || C.xface) ? s2 : 0.0f);
     float F[3] = {0.0f, 0.0f, 0.0f};
     const float kap_l = MET ? shr1(mk) : 1.0f;
     if (HO) {
@@ -248,9 +268,10 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
           p2 = C.xlim ? p2 : 1.0f;
         }
       }
-      float cfl = s2 * C.dtdx;
-      if (MET) cfl = cfl * rcp(0.5f * (kap_l + mk));
-      corr_flux(s2, cfl, p1, p2, w1, w2, F);
+      // 0.5 s (1 - s dt/dxi / kbar); kbar = 1 without metric arrays
+      const float coef = MET ? 0.5f * s2 * fmaf(-s2 * C.dtdx, rcp(0.5f * (kap_l + mk)), 1.0f)
+                             : s2 * fmaf(-C.hdtdx, s2, 0.5f);
+      corr_flux(coef, p1, p2, w1, w2, F);
     }
     const float sg = MET ? mgx * s2 : s2;
     float G[3], HR[3];
@@ -269,7 +290,7 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
   for (int k = 0; k < 3; ++k) N.q[k] = qx[k];
   if (wet) cell<2, true>(qx[0], qx[1], qx[2], C.g, C.dry, N.s, N.f1, N.f2);
   else cell<2, false>(qx[0], qx[1], qx[2], C.g, C.dry, N.s, N.f1, N.f2);
-  if (!wet && (!GEN || r <= C.ny + 1)) R.bady |= __ballot(!(qx[0] > 0.0f) && C.out_lane);
+  if (!wet && (!GEN || r <= C.ny + 1)) R.bady |= __ballot(!(qx[0] > 0.0f)) & C.ymask;
   if (MET) {
     N.kap = mk;
     N.fac = C.dtdy * rcp(mk);
@@ -280,7 +301,7 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
     const float fA[3] = {A.q[2], A.f1, A.f2}, fN[3] = {qx[2], N.f1, N.f2};
     N.s2 = hll(A.q, fA, A.s, qx, fN, N.s, C.eps, N.w1, N.w2);
     const bool in = C.out_lane && (!GEN || (r >= 0 && r <= C.ny + 1));
-    R.smy = fmaxf(R.smy, in ? N.s2 : 0.0f);
+    R.smy = smax2(R.smy, in ? N.s2 : 0.0f);
   } else {
     N.s2 = 0.0f;
 #pragma unroll
@@ -322,9 +343,9 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
       p1 = limit_fast<LIM>(A.w1, N.w1, C.eps);
       p2 = limit_fast<LIM>(A.w2, B.w2, C.eps);
     }
-    float cfl = A.s2 * C.dtdy;
-    if (MET) cfl = cfl * rcp(0.5f * (B.kap + A.kap));
-    corr_flux(A.s2, cfl, p1, p2, A.w1, A.w2, Fa);
+    const float coef = MET ? 0.5f * A.s2 * fmaf(-A.s2 * C.dtdy, rcp(0.5f * (B.kap + A.kap)), 1.0f)
+                           : A.s2 * fmaf(-C.hdtdy, A.s2, 0.5f);
+    corr_flux(coef, p1, p2, A.w1, A.w2, Fa);
   }
   const float sg = MET ? A.gy * A.s2 : A.s2;
   float G[3], H[3];
@@ -344,8 +365,12 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
       source(qn[0], qn[1], qn[2], C.src, C.dry, C.g, C.gdx, C.gdy, C.dt, C.n2, bx, by);
     }
     const int sj = (j + 1 - C.rbase) * C.p4;
+    // an interior tile starts in the unconditional body: its first rows
+    // finish halo rows below the tile, which are not stored
+    if (GEN || r >= C.y0 + 2) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) bstore(qn[k], C.qo[k], C.so, sj);
+      for (int k = 0; k < 3; ++k) bstore(qn[k], C.qo[k], C.so, sj);
+    }
     if (GEN) {
       // xi-boundary ghosts of row j from this row's finished cells (near:
       // column 1 or nx, far: column 2 or nx-1, read as wave-uniform values)
@@ -458,13 +483,16 @@ __global__ void __launch_bounds__(256) k_step(FusedArgs<float> a) {
   }
   C.xface = lane >= 1 && C.i >= 0 && C.i <= C.nx + 1;  // interface i-1/2 in the xi CFL range
   C.xlim = C.i >= 1 && C.i <= C.nx;                   // interface limited
-  C.check_cell = C.i >= -1 && C.i <= C.nx + 1;
+  C.xmask = __ballot(C.i >= -1 && C.i <= C.nx + 1);
+  C.ymask = __ballot(C.out_lane);
   C.src = I.source;
   C.wetdry = a.wetdry;
   for (int k = 0; k < 4; ++k) C.bc[k] = I.bc[k];
   for (int k = 0; k < 8; ++k) C.gr[k] = I.gr[k];
   C.dtdx = (float)(dt / I.dxi);
   C.dtdy = (float)(dt / I.deta);
+  C.hdtdx = 0.5f * C.dtdx;
+  C.hdtdy = 0.5f * C.dtdy;
   C.eps = (float)a.eps;
   C.g = (float)a.g;
   C.dry = (float)a.dry;
@@ -509,37 +537,52 @@ __global__ void __launch_bounds__(256) k_step(FusedArgs<float> a) {
     }
   }
 
-  // Rows y0-2 .. y1+2 in chunks of three.  A chunk runs the unconditional
-  // body when the strip is away from the xi boundaries and all three rows
-  // are past the tile's start-up rows and the domain's bottom ghost rows
-  // (rs) and before the domain's top rows (re); those chunks form one run.
-  // The run is its own loop, entered with no memory operation in flight, so
-  // that the wait for each prefetched row counts only the run's own loads
-  // and stores.
+  // Rows y0-2 .. y1+2 in chunks of three.  A tile away from the domain's
+  // boundaries (interior strip, rows y0-2 .. y1+2 inside 1 .. ny) runs the
+  // unconditional body on every row: its start-up rows compute waves and
+  // fluxes below the tile that are discarded (the store is skipped).  Other
+  // tiles run the general body on the chunks that need a range test and the
+  // unconditional body on the run of chunks in between (rows rs .. re).  An
+  // unconditional run is entered with no memory operation in flight, so that
+  // the wait for each prefetched row counts only the run's own loads and
+  // stores.
   const bool edge = C.x0 == 1 || (C.l_nx >= 3 && C.l_nx <= 61);
-  const int rs = C.y0 + 2 + (C.y0 == 1 ? 2 : 0);
-  const int re = min(C.y1 + 2, C.ny);
   const int rl = C.y1 + 2;
   int r = C.y0 - 2;
-  while (r <= rl) {
-    if (edge || r < rs || r + 2 > re) {
-      row<LIM, HO, MET, PHYS, true>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
-      if (r + 1 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
-      if (r + 2 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 2, Y, Z, X, P2, acc, R, bm, b0, fin1, finm);
-      r += 3;
-      continue;
-    }
+  if (!edge && C.y0 >= 3 && rl <= C.ny) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-    do {
+    for (; r + 2 <= rl; r += 3) {
       row<LIM, HO, MET, PHYS, false>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
       row<LIM, HO, MET, PHYS, false>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
       row<LIM, HO, MET, PHYS, false>(C, r + 2, Y, Z, X, P2, acc, R, bm, b0, fin1, finm);
-      r += 3;
-    } while (r + 2 <= re);
+    }
+    if (r <= rl) row<LIM, HO, MET, PHYS, false>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
+    if (r + 1 <= rl) row<LIM, HO, MET, PHYS, false>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
+  } else {
+    const int rs = C.y0 + 2 + (C.y0 == 1 ? 2 : 0);
+    const int re = min(rl, C.ny);
+    while (r <= rl) {
+      if (edge || r < rs || r + 2 > re) {
+        row<LIM, HO, MET, PHYS, true>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
+        if (r + 1 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
+        if (r + 2 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 2, Y, Z, X, P2, acc, R, bm, b0, fin1, finm);
+        r += 3;
+        continue;
+      }
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+      do {
+        row<LIM, HO, MET, PHYS, false>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
+        row<LIM, HO, MET, PHYS, false>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
+        row<LIM, HO, MET, PHYS, false>(C, r + 2, Y, Z, X, P2, acc, R, bm, b0, fin1, finm);
+        r += 3;
+      } while (r + 2 <= re);
+    }
   }
 
-  const float mx = wave_max(R.smx), my = wave_max(R.smy);
-  tile_finish(S, I, cur, (double)mx, (double)my, R.badx != 0ull, R.bady != 0ull, lane);
+  // wave maxima; a NaN speed (NaN momentum) is reported like a NaN Jacobian
+  const float mx = wave_smax(R.smx), my = wave_smax(R.smy);
+  tile_finish(S, I, cur, (double)mx, (double)my, R.badx != 0ull || mx != mx,
+              R.bady != 0ull || my != my, lane);
 }
 
 template <bool HO, bool MET, bool PHYS>
@@ -565,7 +608,32 @@ static hipError_t launch_ho(int lim, const FusedArgs<float>& a, int blocks, int 
   return launch_lim<true, MET, PHYS>(lim, a, blocks, threads, s);
 }
 
+template <bool MET, bool PHYS>
+static int occupancy_ho(int lim, bool ho, int threads) {
+  int n = 0;
+  hipError_t e;
+  if (!ho) {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step<0, false, MET, PHYS>, threads, 0);
+  } else {
+    switch (lim) {
+      case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step<0, true, MET, PHYS>, threads, 0); break;
+      case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step<1, true, MET, PHYS>, threads, 0); break;
+      case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step<2, true, MET, PHYS>, threads, 0); break;
+      case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step<3, true, MET, PHYS>, threads, 0); break;
+      default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step<4, true, MET, PHYS>, threads, 0); break;
+    }
+  }
+  return e == hipSuccess ? n : 0;
+}
+
 }  // namespace f32
+
+int fused_f32_blocks_per_cu(int lim, bool ho, bool met, bool phys, int threads) {
+  if (met) return phys ? f32::occupancy_ho<true, true>(lim, ho, threads)
+                       : f32::occupancy_ho<true, false>(lim, ho, threads);
+  return phys ? f32::occupancy_ho<false, true>(lim, ho, threads)
+              : f32::occupancy_ho<false, false>(lim, ho, threads);
+}
 
 hipError_t launch_fused_f32(int lim, bool met, bool phys, const FusedArgs<float>& a, int blocks,
                             int threads, hipStream_t s) {

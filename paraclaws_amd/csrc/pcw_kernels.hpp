@@ -48,6 +48,8 @@ struct FusedArgs {
 
 hipError_t launch_fused_f32(int lim, bool met, bool phys, const FusedArgs<float>& a, int blocks,
                             int threads, hipStream_t s);
+// resident blocks per CU of the FP32 fused kernel variant (0 if unknown)
+int fused_f32_blocks_per_cu(int lim, bool ho, bool met, bool phys, int threads);
 hipError_t launch_fused_f64(int lim, bool met, const FusedArgs<double>& a, int blocks,
                             int threads, hipStream_t s);
 hipError_t launch_staged_sweep(const StagedArgs& a, const KInst& hI, int dir, int ho,
