@@ -128,55 +128,47 @@ struct Red {
 template <int DIR, bool WET>
 __device__ __forceinline__ void cell(float h, float hu, float hv, float g, float dry, float& s,
                                      float& f1, float& f2) {
+  // normal velocity un; the transverse flux is un * (transverse momentum)
   const float rh = rcp(h);
-  float u = hu * rh, v = hv * rh;
+  float un = (DIR == 1 ? hu : hv) * rh;
   const float gh = g * h;
   float gc = gh;
   if (WET) {
-    const bool d = !(h > dry);
-    u = d ? 0.0f : u;
-    v = d ? 0.0f : v;
+    un = !(h > dry) ? 0.0f : un;
     gc = gh > 0.0f ? gh : 0.0f;
   }
   const float c = vsqrt(gc);
+  s = fabsf(un) + c;
   if (DIR == 1) {
-    s = fabsf(u) + c;
-    f1 = fmaf(hu, u, gh * h);
-    f2 = hu * v;
+    f1 = fmaf(hu, un, gh * h);
+    f2 = un * hv;
   } else {
-    s = fabsf(v) + c;
-    f1 = hv * u;
-    f2 = fmaf(hv, v, gh * h);
+    f1 = un * hu;
+    f2 = fmaf(hv, un, gh * h);
   }
 }
 
 // HLL with the reference's symmetric speeds s1 = -s2 and regularised
-// denominator eps + s1 - s2; waves are zero when s2 <= eps.  Such
-// interfaces (both cells at rest and dry to within eps) are rare, so the
-// zeroing runs only when a lane of the wave has one; it is a multiplication
-// by 0/1 because the compiler turns a select here into lane branches.
+// denominator D = eps + s1 - s2 (SMOD_approximate_Riemann_solvers.f90:
+// 189-224): with dq = qR - qL and df = fR - fL, the middle state's waves are
+//   w1 = qs - qL = (df - s2 dq - eps qL) / D,  w2 = qR - qs = -(df + s2 dq - eps qR) / D.
+// The eps q terms (below FP32 resolution next to the others, and exactly
+// what the reference leaks into waves of a quiescent state) are dropped, so
+// a state at rest has exactly zero waves.  Waves and speed are zero when
+// s2 <= eps, through a zero reciprocal: no branch.
 __device__ __forceinline__ float hll(const float qL[3], const float fL[3], float sL,
                                      const float qR[3], const float fR[3], float sR, float eps,
                                      float w1[3], float w2[3]) {
-  float s2 = smax2(sL, sR);
-  const float rden = rcp(fmaf(-2.0f, s2, eps));
+  const float s2 = smax2(sL, sR);
+  const bool live = s2 > eps;
+  const float rD = live ? rcp(fmaf(-2.0f, s2, eps)) : 0.0f;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const float t = fmaf(-s2, qR[k] + qL[k], fR[k] - fL[k]);
-    const float qs = t * rden;
-    w1[k] = qs - qL[k];
-    w2[k] = qR[k] - qs;
+    const float dq = qR[k] - qL[k], df = fR[k] - fL[k];
+    w1[k] = fmaf(-s2, dq, df) * rD;
+    w2[k] = -fmaf(s2, dq, df) * rD;
   }
-  if (__builtin_expect(__ballot(!(s2 > eps)) != 0ull, 0)) {
-    const float live = s2 > eps ? 1.0f : 0.0f;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      w1[k] *= live;
-      w2[k] *= live;
-    }
-    s2 *= live;
-  }
-  return s2;
+  return live ? s2 : 0.0f;
 }
 
 // correction flux of one interface: coef (p1 w1 + p2 w2) with
@@ -231,6 +223,9 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
                                     float acc[3], Red& R, float& bm, float& b0, float fin1[3],
                                     float finm[3]) {
   // ---------------------------------------------------------- input row
+  // rows are not interleaved by the scheduler: that would move the
+  // prefetch loads to the end of a chunk and waste their head start
+  __builtin_amdgcn_sched_barrier(0);
   float q[3] = {P.h, P.hu, P.hv};
   const float mk = P.k, mgx = P.gx, mgy = P.gy;
   float bnew = 0.0f;
@@ -366,11 +361,11 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
     }
     const int sj = (j + 1 - C.rbase) * C.p4;
     // an interior tile starts in the unconditional body: its first rows
-    // finish halo rows below the tile, which are not stored
-    if (GEN || r >= C.y0 + 2) {
+    // finish halo rows below the tile, which are not stored (out-of-range
+    // offset, no branch)
+    const unsigned so = (GEN || r >= C.y0 + 2) ? C.so : kOff;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) bstore(qn[k], C.qo[k], C.so, sj);
-    }
+    for (int k = 0; k < 3; ++k) bstore(qn[k], C.qo[k], so, sj);
     if (GEN) {
       // xi-boundary ghosts of row j from this row's finished cells (near:
       // column 1 or nx, far: column 2 or nx-1, read as wave-uniform values)
