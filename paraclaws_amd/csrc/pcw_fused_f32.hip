@@ -214,8 +214,16 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
                                        const Slot& B, float acc[3], float bm, float b0,
                                        float bnew, float fin1[3], float finm[3]);
 
+// Input rows are prefetched kPF rows ahead, through a ring of kPF buffers
+// (6 for the plain variants; 3 where metric arrays or source terms take the
+// registers).
+template <bool MET, bool PHYS>
+struct Ring {
+  static constexpr int kPF = (MET || PHYS) ? 3 : 6;
+};
+
 // One row r of a wave's march.  N: slot of row r (written), A: row r-1, B:
-// row r-2.  P: this row's prefetched input, reloaded with row r+3.
+// row r-2.  P: this row's prefetched input, reloaded with row r+kPF.
 // GEN: the general body with every range test; otherwise the row must be
 // an interior row of an interior strip (see the kernel's chunk test).
 template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
@@ -280,6 +288,18 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
     for (int k = 0; k < 3; ++k) qx[k] = fmaf(-fac, G[k] + HR[k], q[k]);
   }
 
+  // prefetch row r+kPF (clamped to the tile's last row) into this row's
+  // buffer, after the row's last use of it so that the load can write the
+  // same registers; pinned between the sweeps so that the row's loads and
+  // stores are issued apart
+  if (!MET) {
+    __builtin_amdgcn_sched_barrier(0);
+    const int so = (min(r + Ring<MET, PHYS>::kPF, C.y1 + 2) + 1 - C.rbase) * C.p4;
+    P.h = bload(C.qi[0], C.lo, so);
+    P.hu = bload(C.qi[1], C.lo, so);
+    P.hv = bload(C.qi[2], C.lo, so);
+    __builtin_amdgcn_sched_barrier(0);
+  }
   // ---------------------------------------------------------- eta sweep
 #pragma unroll
   for (int k = 0; k < 3; ++k) N.q[k] = qx[k];
@@ -308,19 +328,16 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
     bm = b0;
     b0 = bnew;
   }
-  // prefetch row r+3 (clamped to the tile's last row) into this row's
-  // buffer, after the row's last use of it so that the load can write the
-  // same registers
-  {
-    const int so = (min(r + 3, C.y1 + 2) + 1 - C.rbase) * C.p4;
+  // with metric arrays the prefetch of row r+kPF waits until here: the
+  // metric values of this row are used by the eta sweep
+  if (MET) {
+    const int so = (min(r + Ring<MET, PHYS>::kPF, C.y1 + 2) + 1 - C.rbase) * C.p4;
     P.h = bload(C.qi[0], C.lo, so);
     P.hu = bload(C.qi[1], C.lo, so);
     P.hv = bload(C.qi[2], C.lo, so);
-    if (MET) {
-      P.k = bload(C.mi[0], C.lo, so);
-      P.gx = bload(C.mi[1], C.lo, so);
-      P.gy = bload(C.mi[2], C.lo, so);
-    }
+    P.k = bload(C.mi[0], C.lo, so);
+    P.gx = bload(C.mi[1], C.lo, so);
+    P.gy = bload(C.mi[2], C.lo, so);
   }
 }
 
@@ -427,7 +444,8 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
 }
 
 template <int LIM, bool HO, bool MET, bool PHYS>
-__global__ void __launch_bounds__(256) k_step(FusedArgs<float> a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MET || PHYS) ? 1 : 4)))
+k_step(FusedArgs<float> a) {
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous
@@ -513,65 +531,61 @@ __global__ void __launch_bounds__(256) k_step(FusedArgs<float> a) {
   float fin1[3] = {0.0f, 0.0f, 0.0f}, finm[3] = {0.0f, 0.0f, 0.0f};
   float bm = 0.0f, b0 = 0.0f;
 
-  // prefetch rows y0-2, y0-1, y0
-  Pre P0, P1, P2;
-  {
-    Pre* ps[3] = {&P0, &P1, &P2};
+  // prefetch rows y0-2 .. y0-2+kPF-1 (a tile has at least 7 rows to march)
+  constexpr int kPF = Ring<MET, PHYS>::kPF;
+  Pre P[kPF];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const int so = d * C.p4;  // frame rows y0-1 .. y0+1
-      ps[d]->h = bload(C.qi[0], C.lo, so);
-      ps[d]->hu = bload(C.qi[1], C.lo, so);
-      ps[d]->hv = bload(C.qi[2], C.lo, so);
-      ps[d]->k = ps[d]->gx = ps[d]->gy = 1.0f;
-      if (MET) {
-        ps[d]->k = bload(C.mi[0], C.lo, so);
-        ps[d]->gx = bload(C.mi[1], C.lo, so);
-        ps[d]->gy = bload(C.mi[2], C.lo, so);
-      }
+  for (int d = 0; d < kPF; ++d) {
+    const int so = (min(C.y0 - 2 + d, C.y1 + 2) + 1 - C.rbase) * C.p4;
+    P[d].h = bload(C.qi[0], C.lo, so);
+    P[d].hu = bload(C.qi[1], C.lo, so);
+    P[d].hv = bload(C.qi[2], C.lo, so);
+    P[d].k = P[d].gx = P[d].gy = 1.0f;
+    if (MET) {
+      P[d].k = bload(C.mi[0], C.lo, so);
+      P[d].gx = bload(C.mi[1], C.lo, so);
+      P[d].gy = bload(C.mi[2], C.lo, so);
     }
   }
 
-  // Rows y0-2 .. y1+2 in chunks of three.  A tile away from the domain's
-  // boundaries (interior strip, rows y0-2 .. y1+2 inside 1 .. ny) runs the
-  // unconditional body on every row: its start-up rows compute waves and
-  // fluxes below the tile that are discarded (the store is skipped).  Other
-  // tiles run the general body on the chunks that need a range test and the
-  // unconditional body on the run of chunks in between (rows rs .. re).  An
-  // unconditional run is entered with no memory operation in flight, so that
-  // the wait for each prefetched row counts only the run's own loads and
-  // stores.
+  // Rows y0-2 .. y1+2 in chunks of six (the slots rotate every three rows,
+  // the input buffers every six).  A chunk runs the unconditional body when
+  // all its rows satisfy every range condition: for a tile away from the
+  // domain's boundaries (interior strip, rows y0-2 .. y1+2 inside 1 .. ny)
+  // that is every full chunk, whose start-up rows compute waves and fluxes
+  // below the tile that are discarded (the store is skipped); for other
+  // tiles, the chunks between the tile's start-up rows and the domain's
+  // bottom ghost rows (rs) and its top rows (re), away from the xi
+  // boundaries.  Other chunks run the general body.  A run of unconditional
+  // chunks is entered with no memory operation in flight, so that the wait
+  // for each prefetched row counts only the run's own loads and stores.
   const bool edge = C.x0 == 1 || (C.l_nx >= 3 && C.l_nx <= 61);
   const int rl = C.y1 + 2;
+  const bool interior = !edge && C.y0 >= 3 && rl <= C.ny;
+  const int rs = interior ? C.y0 - 2 : C.y0 + 2 + (C.y0 == 1 ? 2 : 0);
+  const int re = interior ? rl : min(rl, C.ny);
   int r = C.y0 - 2;
-  if (!edge && C.y0 >= 3 && rl <= C.ny) {
+  while (r <= rl) {
+    if (edge || r < rs || r + 5 > re) {
+      row<LIM, HO, MET, PHYS, true>(C, r, X, Y, Z, P[0 % kPF], acc, R, bm, b0, fin1, finm);
+      if (r + 1 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 1, Z, X, Y, P[1 % kPF], acc, R, bm, b0, fin1, finm);
+      if (r + 2 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 2, Y, Z, X, P[2 % kPF], acc, R, bm, b0, fin1, finm);
+      if (r + 3 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 3, X, Y, Z, P[3 % kPF], acc, R, bm, b0, fin1, finm);
+      if (r + 4 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 4, Z, X, Y, P[4 % kPF], acc, R, bm, b0, fin1, finm);
+      if (r + 5 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 5, Y, Z, X, P[5 % kPF], acc, R, bm, b0, fin1, finm);
+      r += 6;
+      continue;
+    }
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-    for (; r + 2 <= rl; r += 3) {
-      row<LIM, HO, MET, PHYS, false>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
-      row<LIM, HO, MET, PHYS, false>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
-      row<LIM, HO, MET, PHYS, false>(C, r + 2, Y, Z, X, P2, acc, R, bm, b0, fin1, finm);
-    }
-    if (r <= rl) row<LIM, HO, MET, PHYS, false>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
-    if (r + 1 <= rl) row<LIM, HO, MET, PHYS, false>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
-  } else {
-    const int rs = C.y0 + 2 + (C.y0 == 1 ? 2 : 0);
-    const int re = min(rl, C.ny);
-    while (r <= rl) {
-      if (edge || r < rs || r + 2 > re) {
-        row<LIM, HO, MET, PHYS, true>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
-        if (r + 1 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
-        if (r + 2 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 2, Y, Z, X, P2, acc, R, bm, b0, fin1, finm);
-        r += 3;
-        continue;
-      }
-      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-      do {
-        row<LIM, HO, MET, PHYS, false>(C, r, X, Y, Z, P0, acc, R, bm, b0, fin1, finm);
-        row<LIM, HO, MET, PHYS, false>(C, r + 1, Z, X, Y, P1, acc, R, bm, b0, fin1, finm);
-        row<LIM, HO, MET, PHYS, false>(C, r + 2, Y, Z, X, P2, acc, R, bm, b0, fin1, finm);
-        r += 3;
-      } while (r + 2 <= re);
-    }
+    do {
+      row<LIM, HO, MET, PHYS, false>(C, r, X, Y, Z, P[0 % kPF], acc, R, bm, b0, fin1, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 1, Z, X, Y, P[1 % kPF], acc, R, bm, b0, fin1, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 2, Y, Z, X, P[2 % kPF], acc, R, bm, b0, fin1, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 3, X, Y, Z, P[3 % kPF], acc, R, bm, b0, fin1, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 4, Z, X, Y, P[4 % kPF], acc, R, bm, b0, fin1, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 5, Y, Z, X, P[5 % kPF], acc, R, bm, b0, fin1, finm);
+      r += 6;
+    } while (r + 5 <= re);
   }
 
   // wave maxima; a NaN speed (NaN momentum) is reported like a NaN Jacobian
