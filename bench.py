@@ -38,6 +38,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE_F32 = 24.0
 BYTES_PER_CELL_UPDATE_F64 = 48.0
+# the FP32 fused step kernel of this workload (MC limiter, second order, no
+# metric arrays, no source terms)
+KERNEL_F32 = "pcw::f32::k_step<4, true, false, false>"
 METRIC = "cell-updates/s/GPU (FP32 batched vs FP64 plain) at stated error; % HBM peak"
 
 
@@ -241,11 +244,13 @@ def cpu_baseline(n, target_s=15.0):
 
 def load_traffic(n):
     """HBM bytes per launch of the fused FP32 step from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE runs, if they exist for this grid size."""
+    FETCH_SIZE/WRITE_SIZE runs, if they exist for this grid size and kernel."""
     p = os.path.join(ROOT, "profiles", f"pmc_fused_step_f32_{n}.json")
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
+            rec = json.load(f)
+        if rec.get("kernel") == KERNEL_F32:
+            return rec.get("traffic_bytes_per_launch")
     return None
 
 
@@ -377,7 +382,7 @@ def main():
                    "tile_rows_waves": r32["tuned"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_fused_step<float>", "kernel_avg_ms": kernel_ms,
+                     "kernel": KERNEL_F32, "kernel_avg_ms": kernel_ms,
                      "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_F32},
         "cpu_baseline": cpu,
     }
