@@ -222,26 +222,16 @@ struct Ring {
   static constexpr int kPF = (MET || PHYS) ? 3 : 6;
 };
 
-// One row r of a wave's march.  N: slot of row r (written), A: row r-1, B:
-// row r-2.  P: this row's prefetched input, reloaded with row r+kPF.
-// GEN: the general body with every range test; otherwise the row must be
-// an interior row of an interior strip (see the kernel's chunk test).
+// xi sweep of row r: the xi-updated state qx of the row's cells from the
+// prefetched input P (P itself for rows outside 1..ny in the general body).
 template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
-__device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot& B, Pre& P,
-                                    float acc[3], Red& R, float& bm, float& b0, float fin1[3],
-                                    float finm[3]) {
-  // ---------------------------------------------------------- input row
-  // rows are not interleaved by the scheduler: that would move the
-  // prefetch loads to the end of a chunk and waste their head start
-  __builtin_amdgcn_sched_barrier(0);
-  float q[3] = {P.h, P.hu, P.hv};
-  const float mk = P.k, mgx = P.gx, mgy = P.gy;
-  float bnew = 0.0f;
-  if (PHYS && C.has_b && r >= C.y0) bnew = bload(C.bi, C.lo, (r - C.rbase) * C.p4);  // b(i, r-1)
+__device__ __forceinline__ void xi_row(const Ctx& C, int r, const Pre& P, Red& R, float qx[3]) {
+  const float q[3] = {P.h, P.hu, P.hv};
+  const float mk = P.k, mgx = P.gx;
   const bool wet = PHYS && C.wetdry;
-
-  // ---------------------------------------------------------- xi sweep
-  float qx[3] = {q[0], q[1], q[2]};
+  qx[0] = q[0];
+  qx[1] = q[1];
+  qx[2] = q[2];
   if (!GEN || (r >= 1 && r <= C.ny)) {
     float s, f1, f2;
     if (wet) cell<1, true>(q[0], q[1], q[2], C.g, C.dry, s, f1, f2);
@@ -287,20 +277,31 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
 #pragma unroll
     for (int k = 0; k < 3; ++k) qx[k] = fmaf(-fac, G[k] + HR[k], q[k]);
   }
+}
 
-  // prefetch row r+kPF (clamped to the tile's last row) into this row's
-  // buffer, after the row's last use of it so that the load can write the
-  // same registers; pinned between the sweeps so that the row's loads and
-  // stores are issued apart
-  if (!MET) {
-    __builtin_amdgcn_sched_barrier(0);
-    const int so = (min(r + Ring<MET, PHYS>::kPF, C.y1 + 2) + 1 - C.rbase) * C.p4;
-    P.h = bload(C.qi[0], C.lo, so);
-    P.hu = bload(C.qi[1], C.lo, so);
-    P.hv = bload(C.qi[2], C.lo, so);
-    __builtin_amdgcn_sched_barrier(0);
+// load row r+kPF (clamped to the tile's last row) into P
+template <bool MET, bool PHYS>
+__device__ __forceinline__ void prefetch(const Ctx& C, int r, Pre& P) {
+  const int so = (min(r + Ring<MET, PHYS>::kPF, C.y1 + 2) + 1 - C.rbase) * C.p4;
+  P.h = bload(C.qi[0], C.lo, so);
+  P.hu = bload(C.qi[1], C.lo, so);
+  P.hv = bload(C.qi[2], C.lo, so);
+  if (MET) {
+    P.k = bload(C.mi[0], C.lo, so);
+    P.gx = bload(C.mi[1], C.lo, so);
+    P.gy = bload(C.mi[2], C.lo, so);
   }
-  // ---------------------------------------------------------- eta sweep
+}
+
+// eta sweep of row r from its xi-updated state qx: the row's eta cell
+// values into slot N, the face below it (rows r-1 | r), and from row y0+1
+// on the finishing of face r-1 (finish).  mk, mgy: the row's metric values.
+template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
+__device__ __forceinline__ void eta_row(const Ctx& C, int r, const float qx[3], float mk,
+                                        float mgy, Slot& N, Slot& A, Slot& B, float acc[3],
+                                        Red& R, float& bm, float& b0, float bnew, float fin1[3],
+                                        float finm[3]) {
+  const bool wet = PHYS && C.wetdry;
 #pragma unroll
   for (int k = 0; k < 3; ++k) N.q[k] = qx[k];
   if (wet) cell<2, true>(qx[0], qx[1], qx[2], C.g, C.dry, N.s, N.f1, N.f2);
@@ -328,17 +329,35 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
     bm = b0;
     b0 = bnew;
   }
-  // with metric arrays the prefetch of row r+kPF waits until here: the
-  // metric values of this row are used by the eta sweep
-  if (MET) {
-    const int so = (min(r + Ring<MET, PHYS>::kPF, C.y1 + 2) + 1 - C.rbase) * C.p4;
-    P.h = bload(C.qi[0], C.lo, so);
-    P.hu = bload(C.qi[1], C.lo, so);
-    P.hv = bload(C.qi[2], C.lo, so);
-    P.k = bload(C.mi[0], C.lo, so);
-    P.gx = bload(C.mi[1], C.lo, so);
-    P.gy = bload(C.mi[2], C.lo, so);
+}
+
+// One row r of a wave's march, sweeps in sequence.  N: slot of row r
+// (written), A: row r-1, B: row r-2.  P: this row's prefetched input,
+// reloaded with row r+kPF.  GEN: the general body with every range test;
+// otherwise the row must be an interior row of an interior strip (see the
+// kernel's chunk test).
+template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
+__device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot& B, Pre& P,
+                                    float acc[3], Red& R, float& bm, float& b0, float fin1[3],
+                                    float finm[3]) {
+  // rows are not interleaved by the scheduler: that would move the
+  // prefetch loads to the end of a chunk and waste their head start
+  __builtin_amdgcn_sched_barrier(0);
+  const float mk = P.k, mgy = P.gy;
+  float bnew = 0.0f;
+  if (PHYS && C.has_b && r >= C.y0) bnew = bload(C.bi, C.lo, (r - C.rbase) * C.p4);  // b(i, r-1)
+  float qx[3];
+  xi_row<LIM, HO, MET, PHYS, GEN>(C, r, P, R, qx);
+  // the prefetch goes after the row's last use of P, so that the load can
+  // write the same registers; without metric arrays it is pinned between
+  // the sweeps so that the row's loads and stores are issued apart
+  if (!MET) {
+    __builtin_amdgcn_sched_barrier(0);
+    prefetch<MET, PHYS>(C, r, P);
+    __builtin_amdgcn_sched_barrier(0);
   }
+  eta_row<LIM, HO, MET, PHYS, GEN>(C, r, qx, mk, mgy, N, A, B, acc, R, bm, b0, bnew, fin1, finm);
+  if (MET) prefetch<MET, PHYS>(C, r, P);  // the eta sweep used this row's metric values
 }
 
 // The part of a row that needs interface r-1 (from row y0+1 on): its
