@@ -54,6 +54,8 @@ program paraCLAWs_gpu
     call write_base_info(mesh)
     call write_mesh_to_disk(decomposition, mesh)
     call writeData(decomposition, t_ctrl%time, solution)
+    call write_vtk_rank(decomposition, mesh, solution, t_ctrl%time)
+    call write_pvts_rank(decomposition, t_ctrl%time)
     call device_upload(solution%vars)
     call update_rank_boundaries(decomposition, mesh, solution)
     call device_totals(totals)
@@ -84,6 +86,8 @@ program paraCLAWs_gpu
             call device_download(solution%vars)
             call execute_post(decomposition, mesh, solution)
             call writeData(decomposition, t_ctrl%time, solution)
+            call write_vtk_rank(decomposition, mesh, solution, t_ctrl%time)
+            call write_pvts_rank(decomposition, t_ctrl%time)
             write(*,1) t_ctrl%time, t_ctrl%get_previous_dt(), t_ctrl%get_CFL(), t_ctrl%get_dt(), &
                 t_ctrl%get_snapshot_index()
             call t_ctrl%advance_snapshot_index()

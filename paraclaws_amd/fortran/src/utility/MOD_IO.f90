@@ -2,9 +2,18 @@
 ! the reference's 28-line order; custom.in (written but unread by the
 ! reference) carries this host's device settings; snapshots use the
 ! reference's qout.dat/aout.dat format (50e26.16, j outer, i inner, ghosts
-! included) under output/<time>/rank_0/.
+! included) under output/<time>/rank_0/.  Built with USE_VTK (the Makefile's
+! use_vtk=1, the default as in the reference), each snapshot also goes to the
+! reference's VTK layout (MOD_IO.f90:111-394 there): a StructuredGrid piece
+! output/<time>/rank_0/solution_rank_0.vts, the parallel header
+! output/<time>/solution.pvts and the collection output/solution.pvd.  The
+! XML is written here directly (the reference links the VTKFortran library).
 module MOD_IO
+    use iso_fortran_env, only: int8, int32
     implicit none
+    double precision, allocatable, private :: pvd_times(:)
+    character(len=64), parameter, private :: b64_table = &
+        'ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/'
 contains
 
     subroutine read_solver_control(t_ctrl, mesh, solution)
@@ -133,6 +142,199 @@ contains
             end do
         end do
         close(unit=1)
+    end subroutine
+
+    subroutine write_vtk_rank(decomposition, mesh, solution, time)
+        !! StructuredGrid piece of this rank's interior cells (reference
+        !! write_vtk_rank): points are the cell centres (x, y, 0); point data
+        !! q1..q(num_equations) in Float64; VTK "binary" DataArrays (base64 of
+        !! a UInt32 byte count, then base64 of the little-endian values).
+        use MOD_decomposition, only: decomp_info
+        use MOD_domain, only: mesh_fields, solution_fields
+        use MOD_solver_parameters, only: num_equations
+        type(decomp_info), intent(in) :: decomposition
+        type(mesh_fields), intent(in) :: mesh
+        type(solution_fields), intent(in) :: solution
+        double precision, intent(in) :: time
+#ifdef USE_VTK
+        integer :: ilow, ihigh, jlow, jhigh, i, j, k, m, npts, u
+        double precision, allocatable :: buf(:)
+        character(len=40) :: timestamp_str, qname
+        character(len=200) :: dirname, filename
+        character(len=96) :: ext
+        character(len=1), parameter :: nl = new_line('a')
+
+        call decomposition%get_local_block_bounds(ilow, ihigh, jlow, jhigh)
+        npts = (ihigh - ilow + 1) * (jhigh - jlow + 1)
+        write(ext, '(I0,1X,I0,1X,I0,1X,I0,A)') ilow, ihigh, jlow, jhigh, ' 1 1'
+        write(timestamp_str, '(F20.10)') time
+        dirname = 'output/'//trim(adjustl(timestamp_str))//'/rank_0'
+        call execute_command_line('mkdir -p '//trim(dirname))
+        filename = trim(dirname)//'/solution_rank_0.vts'
+        open(newunit=u, file=trim(filename), status='replace', access='stream', form='unformatted')
+        write(u) '<?xml version="1.0"?>'//nl
+        write(u) '<VTKFile type="StructuredGrid" version="1.0" byte_order="LittleEndian" ' // &
+            'header_type="UInt32">'//nl
+        write(u) '  <StructuredGrid WholeExtent="'//trim(ext)//'">'//nl
+        write(u) '    <Piece Extent="'//trim(ext)//'">'//nl
+        write(u) '      <Points>'//nl
+        write(u) '        <DataArray type="Float64" NumberOfComponents="3" Name="Points" ' // &
+            'format="binary">'//nl
+        allocate(buf(3 * npts))
+        k = 0
+        do j = jlow, jhigh
+            do i = ilow, ihigh
+                buf(k + 1) = mesh%physical_space%x(i, j)
+                buf(k + 2) = mesh%physical_space%y(i, j)
+                buf(k + 3) = 0.0d0
+                k = k + 3
+            end do
+        end do
+        call write_b64_array(u, buf)
+        deallocate(buf)
+        write(u) nl//'        </DataArray>'//nl
+        write(u) '      </Points>'//nl
+        write(u) '      <PointData>'//nl
+        allocate(buf(npts))
+        do m = 1, num_equations
+            k = 0
+            do j = jlow, jhigh
+                do i = ilow, ihigh
+                    k = k + 1
+                    buf(k) = solution%vars(m, i, j)
+                end do
+            end do
+            write(qname, '(A,I0)') 'q', m
+            write(u) '        <DataArray type="Float64" NumberOfComponents="1" Name="' // &
+                trim(qname)//'" format="binary">'//nl
+            call write_b64_array(u, buf)
+            write(u) nl//'        </DataArray>'//nl
+        end do
+        deallocate(buf)
+        write(u) '      </PointData>'//nl
+        write(u) '    </Piece>'//nl
+        write(u) '  </StructuredGrid>'//nl
+        write(u) '</VTKFile>'//nl
+        close(u)
+#endif
+    end subroutine
+
+    subroutine write_pvts_rank(decomposition, time)
+        !! Parallel header of the snapshot (one piece per rank; one rank here)
+        !! and the time-series collection output/solution.pvd (reference
+        !! write_pvts_rank / write_pvd_collection).
+        use MOD_decomposition, only: decomp_info
+        use MOD_solver_parameters, only: num_equations
+        type(decomp_info), intent(in) :: decomposition
+        double precision, intent(in) :: time
+#ifdef USE_VTK
+        integer :: ilow, ihigh, jlow, jhigh, m, u, count, idx
+        double precision, allocatable :: times(:)
+        character(len=40) :: timestamp_str, qname
+        character(len=96) :: ext
+
+        call decomposition%get_local_block_bounds(ilow, ihigh, jlow, jhigh)
+        write(ext, '(I0,1X,I0,1X,I0,1X,I0,A)') ilow, ihigh, jlow, jhigh, ' 1 1'
+        write(timestamp_str, '(F20.10)') time
+        open(newunit=u, file='output/'//trim(adjustl(timestamp_str))//'/solution.pvts', &
+            status='replace', form='formatted')
+        write(u, '(A)') '<?xml version="1.0"?>'
+        write(u, '(A)') '<VTKFile type="PStructuredGrid" version="1.0" byte_order="LittleEndian">'
+        write(u, '(A)') '  <PStructuredGrid WholeExtent="'//trim(ext)//'" GhostLevel="0">'
+        write(u, '(A)') '    <PPointData>'
+        do m = 1, num_equations
+            write(qname, '(A,I0)') 'q', m
+            write(u, '(A)') '      <PDataArray type="Float64" Name="'//trim(qname)// &
+                '" NumberOfComponents="1"/>'
+        end do
+        write(u, '(A)') '    </PPointData>'
+        write(u, '(A)') '    <PPoints>'
+        write(u, '(A)') '      <PDataArray type="Float64" NumberOfComponents="3" Name="Points"/>'
+        write(u, '(A)') '    </PPoints>'
+        write(u, '(A)') '    <Piece Extent="'//trim(ext)//'" Source="rank_0/solution_rank_0.vts"/>'
+        write(u, '(A)') '  </PStructuredGrid>'
+        write(u, '(A)') '</VTKFile>'
+        close(u)
+
+        ! collection: one DataSet per snapshot time, rewritten each time
+        count = 0
+        if (allocated(pvd_times)) count = size(pvd_times)
+        if (count > 0) then
+            if (pvd_times(count) == time) return
+        end if
+        allocate(times(count + 1))
+        if (count > 0) times(1:count) = pvd_times
+        times(count + 1) = time
+        call move_alloc(times, pvd_times)
+        open(newunit=u, file='output/solution.pvd', status='replace', form='formatted')
+        write(u, '(A)') '<?xml version="1.0"?>'
+        write(u, '(A)') '<VTKFile type="Collection" version="1.0" byte_order="LittleEndian">'
+        write(u, '(A)') '  <Collection>'
+        do idx = 1, count + 1
+            write(timestamp_str, '(F20.10)') pvd_times(idx)
+            write(u, '(A)') '    <DataSet timestep="'//trim(adjustl(timestamp_str))// &
+                '" group="" part="0" file="'//trim(adjustl(timestamp_str))//'/solution.pvts"/>'
+        end do
+        write(u, '(A)') '  </Collection>'
+        write(u, '(A)') '</VTKFile>'
+        close(u)
+#endif
+    end subroutine
+
+    subroutine write_b64_array(u, a)
+        !! One VTK inline binary DataArray body: base64 of the UInt32 byte
+        !! count, then base64 of the values' bytes (encoded in chunks of a
+        !! multiple of three bytes, so the chunks concatenate to one encoding).
+        integer, intent(in) :: u
+        double precision, intent(in) :: a(:)
+        integer(int8) :: hdr(4)
+        integer(int8), allocatable :: bytes(:)
+        integer :: n, first, last
+        integer, parameter :: chunk = 3 * 131072
+        n = 8 * size(a)
+        hdr = transfer(int(n, int32), hdr)
+        call write_b64(u, hdr)
+        allocate(bytes(n))
+        bytes = transfer(a, bytes)
+        do first = 1, n, chunk
+            last = min(n, first + chunk - 1)
+            call write_b64(u, bytes(first:last))
+        end do
+        deallocate(bytes)
+    end subroutine
+
+    subroutine write_b64(u, bytes)
+        !! Base64 (RFC 4648, with '=' padding) of a byte string.
+        integer, intent(in) :: u
+        integer(int8), intent(in) :: bytes(:)
+        character(len=:), allocatable :: out
+        integer :: n, i, k, b0, b1, b2
+        n = size(bytes)
+        allocate(character(len=4 * ((n + 2) / 3)) :: out)
+        k = 0
+        do i = 1, n, 3
+            b0 = iand(int(bytes(i)), 255)
+            b1 = 0
+            b2 = 0
+            if (i + 1 <= n) b1 = iand(int(bytes(i + 1)), 255)
+            if (i + 2 <= n) b2 = iand(int(bytes(i + 2)), 255)
+            out(k+1:k+1) = b64_table(ishft(b0, -2) + 1:ishft(b0, -2) + 1)
+            out(k+2:k+2) = b64_table(ior(ishft(iand(b0, 3), 4), ishft(b1, -4)) + 1: &
+                                     ior(ishft(iand(b0, 3), 4), ishft(b1, -4)) + 1)
+            if (i + 1 <= n) then
+                out(k+3:k+3) = b64_table(ior(ishft(iand(b1, 15), 2), ishft(b2, -6)) + 1: &
+                                         ior(ishft(iand(b1, 15), 2), ishft(b2, -6)) + 1)
+            else
+                out(k+3:k+3) = '='
+            end if
+            if (i + 2 <= n) then
+                out(k+4:k+4) = b64_table(iand(b2, 63) + 1:iand(b2, 63) + 1)
+            else
+                out(k+4:k+4) = '='
+            end if
+            k = k + 4
+        end do
+        write(u) out
     end subroutine
 
 end module MOD_IO

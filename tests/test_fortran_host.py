@@ -70,6 +70,63 @@ def test_host_builds_with_example_override():
     assert os.path.exists(EXE)
 
 
+def read_vts(path):
+    """Point arrays and extent of a VTK StructuredGrid file with inline
+    "binary" DataArrays (base64 UInt32 byte count, then base64 data)."""
+    import base64
+    import re
+
+    txt = open(path).read()
+    out = {"extent": [int(v) for v in re.search(r'<Piece Extent="([^"]+)"', txt).group(1).split()]}
+    for name, body in re.findall(r'<DataArray[^>]*Name="([^"]+)"[^>]*format="binary">\s*([A-Za-z0-9+/=]+)\s*</DataArray>', txt):
+        nbytes = int(np.frombuffer(base64.b64decode(body[:8]), dtype="<u4")[0])
+        data = base64.b64decode(body[8:])
+        assert len(data) == nbytes
+        out[name] = np.frombuffer(data, dtype="<f8")
+    return out
+
+
+def test_vtk_base64_writer(tmp_path):
+    """MOD_IO's base64 writer against Python's, for every length mod 3, and
+    a DataArray body (UInt32 byte count encoded apart from the data)."""
+    import base64
+
+    build = os.path.join(FDIR, "build")
+    if not os.path.exists(os.path.join(build, "mod", "mod_io.mod")):
+        pytest.skip("Fortran host not built (run __graft_entry__.build())")
+    (tmp_path / "t.f90").write_text("""program t
+    use MOD_IO
+    use iso_fortran_env, only: int8
+    integer(int8) :: b(7) = [0_int8, -1_int8, 127_int8, -128_int8, 64_int8, 3_int8, -77_int8]
+    double precision :: a(3) = [1.5d0, -2.25d0, 1d300]
+    integer :: u, n
+    open(newunit=u, file='out.txt', access='stream', form='unformatted', status='replace')
+    do n = 0, 7
+        call write_b64(u, b(1:n))
+        write(u) new_line('a')
+    end do
+    call write_b64_array(u, a)
+    close(u)
+end program
+""")
+    objs = [os.path.join(build, "obj", f) for f in sorted(os.listdir(os.path.join(build, "obj")))
+            if f.endswith(".o") and f != "main.o"]
+    lib = os.path.join(ROOT, "paraclaws_amd")
+    fc = "/opt/rocm/lib/llvm/bin/flang"
+    r = subprocess.run([fc, "-cpp", "-module-dir", str(tmp_path), "-I", os.path.join(build, "mod"),
+                        "t.f90", *objs, f"-L{lib}", "-lpcw_hip", f"-Wl,-rpath,{lib}", "-o", "t"],
+                       cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run(["./t"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    lines = (tmp_path / "out.txt").read_text().split("\n")
+    raw = bytes([0, 255, 127, 128, 64, 3, 256 - 77])
+    for n in range(8):
+        assert lines[n] == base64.b64encode(raw[:n]).decode()
+    a = np.array([1.5, -2.25, 1e300], dtype="<f8").tobytes()
+    assert lines[8] == (base64.b64encode(np.uint32(len(a)).tobytes()) + base64.b64encode(a)).decode()
+
+
 def _control_in(n, t_final, limiter=4, bc=(4, 4, 4, 4), cfl=0.9, dt0=1e-10, ho=True, lim=True):
     vals = [3, 2, 2, 10, 2, 0, t_final, 10, dt0, ".true.", cfl, 1e-10, 1e-2, 1.2, 0.5,
             ".true." if ho else ".false.", ".true." if lim else ".false.", limiter, n, n, -1, 1, -1, 1,
@@ -111,6 +168,20 @@ def test_fortran_host_run_matches_oracle(oracle, tmp_path, precision, engine):
     qout = np.loadtxt(tmp_path / "output" / times[-1] / "rank_0" / "qout.dat")
     assert qout.shape == ((n + 4) * (n + 4), 3)
     assert np.allclose(qout.reshape(n + 4, n + 4, 3)[m], q[m], rtol=1e-15, atol=1e-16)
+    # VTK snapshots in the reference's layout: the piece holds the interior
+    # cell centres and q1..q3 exactly; the parallel header names the piece;
+    # the collection lists every snapshot
+    piece = read_vts(tmp_path / "output" / times[-1] / "rank_0" / "solution_rank_0.vts")
+    assert piece["extent"] == [1, n, 1, n, 1, 1]
+    xc = -1.0 + (np.arange(n) + 0.5) * (2.0 / n)
+    pts = piece["Points"].reshape(n, n, 3)
+    assert np.allclose(pts[..., 0], xc[None, :], atol=1e-15) and np.allclose(pts[..., 1], xc[:, None], atol=1e-15)
+    for k in range(3):
+        assert np.array_equal(piece[f"q{k + 1}"].reshape(n, n), q[2:-2, 2:-2, k])
+    pvts = (tmp_path / "output" / times[-1] / "solution.pvts").read_text()
+    assert 'Source="rank_0/solution_rank_0.vts"' in pvts and 'Name="q3"' in pvts
+    pvd = (tmp_path / "output" / "solution.pvd").read_text()
+    assert pvd.count("<DataSet ") == 11 and f'file="{times[-1]}/solution.pvts"' in pvd
 
 
 @pytest.mark.gpu
