@@ -7,8 +7,9 @@
 //     J*q in BLAS column order, true IEEE divisions, reference-rounded mesh
 //     metrics), so the FP64 mode reproduces the CPU restatement bit for bit.
 //     Every translation unit is compiled with -ffp-contract=off.
-//   Fast (float): the closed forms, reciprocal-based division and explicit
-//     fmaf, unit metrics (uniform Cartesian mesh).
+//   Fast (float): the limiter's closed forms with reciprocal-based division
+//     and explicit fmaf, used by the FP32 step kernel (which holds its own
+//     cell and HLL arithmetic).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -117,37 +118,6 @@ __device__ __forceinline__ CellQ<double> cell_exact(double h, double hu, double 
   return r;
 }
 
-// ---------------------------------------------------------- Fast (float)
-template <int DIR, bool WETDRY>
-__device__ __forceinline__ CellQ<float> cell_fast(float h, float hu, float hv, float g,
-                                                  float dry) {
-  CellQ<float> r;
-  r.bad = 0;
-  float rh = __builtin_amdgcn_rcpf(h);
-  float u = hu * rh, v = hv * rh;
-  float gh = g * h;
-  float gc = gh;
-  if (WETDRY) {
-    bool d = !(h > dry);
-    u = d ? 0.0f : u;
-    v = d ? 0.0f : v;
-    gc = gh > 0.0f ? gh : 0.0f;
-  }
-  float c = __builtin_amdgcn_sqrtf(gc);  // v_sqrt_f32 (1 ulp), not the correctly rounded expansion
-  float un = (DIR == 1) ? u : v;
-  r.s = fabsf(un) + c;
-  if (DIR == 1) {
-    r.f0 = hu;
-    r.f1 = fmaf(hu, u, gh * h);
-    r.f2 = hu * v;
-  } else {
-    r.f0 = hv;
-    r.f1 = hv * u;
-    r.f2 = fmaf(hv, v, gh * h);
-  }
-  return r;
-}
-
 // ----------------------------------------------------------------- waves
 // HLL of SMOD_approximate_Riemann_solvers.f90:189-224 for L | R.
 template <typename T>
@@ -177,25 +147,6 @@ __device__ __forceinline__ Waves<double> hll_exact(const double qL[3], const Cel
 #pragma unroll
     for (int k = 0; k < 3; ++k) { w.w1[k] = 0.0; w.w2[k] = 0.0; }
   }
-  return w;
-}
-
-__device__ __forceinline__ Waves<float> hll_fast(const float qL[3], const CellQ<float>& cL,
-                                                 const float qR[3], const CellQ<float>& cR,
-                                                 float eps) {
-  Waves<float> w;
-  float s2 = fmaxf(cL.s, cR.s);
-  bool live = s2 > eps;
-  float rden = __builtin_amdgcn_rcpf((eps - s2) - s2);
-  float fL[3] = {cL.f0, cL.f1, cL.f2}, fR[3] = {cR.f0, cR.f1, cR.f2};
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float t = fmaf(-s2, qR[k] + qL[k], fR[k] - fL[k]);
-    float qs = t * rden;
-    w.w1[k] = live ? qs - qL[k] : 0.0f;
-    w.w2[k] = live ? qR[k] - qs : 0.0f;
-  }
-  w.s2 = live ? s2 : 0.0f;
   return w;
 }
 

@@ -16,8 +16,6 @@
 // second_order_flux_correction, integrator_flux_correction_increment, the
 // CFL max (as max wave speed), integrate_source_term and
 // apply_physical_boundaries.
-#include <type_traits>
-
 #include "pcw_device.hpp"
 #include "pcw_engine.hpp"
 #include "pcw_kernels.hpp"
@@ -25,11 +23,9 @@
 
 namespace pcw {
 
-template <typename T>
-__device__ __forceinline__ T ld(const T* p) { return *p; }
-
-template <typename T, bool EXACT, int LIM, bool MET>
-__global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
+template <int LIM, bool MET>
+__global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
+  using T = double;
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous
@@ -61,8 +57,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   const bool check_cell = i >= -1 && i <= nx + 1;
 
   const double dtod_x = dt / I.dxi, dtod_y = dt / I.deta;
-  const T dtdx = (T)dtod_x, dtdy = (T)dtod_y;
-  const T eps = (T)a.eps, g = (T)a.g, dry = (T)a.dry;
+  const T g = a.g, dry = a.dry;
   Phys P;
   P.g = a.g;
   P.eps = a.eps;
@@ -81,9 +76,8 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
   const int l_nx = nx - x0 + 2;                // lane of column nx
   const bool do_left = x0 == 1;
   const bool do_right = l_nx >= 3 && l_nx <= 61;
-  // metric scalars: MT is double in the exact mode, T otherwise
-  using MT = typename std::conditional<EXACT, double, T>::type;
-  const MT gam_eta_col = EXACT ? (MT)gamma_eta_exact(M, i) : MT(1);
+  using MT = double;
+  const MT gam_eta_col = gamma_eta_exact(M, i);
 
   T smx = T(0), smy = T(0);
   int bad = 0;   // bit 1: NaN/complex in the xi sweep, bit 2: in the eta sweep
@@ -105,12 +99,12 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
 
   // prefetch first row
   long long roff = (long long)(y0 - 2 + 1) * pitch + fc;
-  T nh = ld(qin + roff), nhu = ld(qin + plane + roff), nhv = ld(qin + 2 * plane + roff);
+  T nh = qin[roff], nhu = qin[plane + roff], nhv = qin[2 * plane + roff];
   T nk = T(1), ngx = T(1), ngy = T(1);
   if (MET) {
-    nk = ld(mq + roff);
-    ngx = ld(mq + plane + roff);
-    ngy = ld(mq + 2 * plane + roff);
+    nk = mq[roff];
+    ngx = mq[plane + roff];
+    ngy = mq[2 * plane + roff];
   }
 
   for (int r = y0 - 2; r <= y1 + 2; ++r) {
@@ -118,30 +112,26 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
     const T mk_r = nk, mgx_r = ngx, mgy_r = ngy;
     if (r < y1 + 2) {  // prefetch the next row
       long long o = (long long)(r + 2) * pitch + fc;
-      nh = ld(qin + o);
-      nhu = ld(qin + plane + o);
-      nhv = ld(qin + 2 * plane + o);
+      nh = qin[o];
+      nhu = qin[plane + o];
+      nhv = qin[2 * plane + o];
       if (MET) {
-        nk = ld(mq + o);
-        ngx = ld(mq + plane + o);
-        ngy = ld(mq + 2 * plane + o);
+        nk = mq[o];
+        ngx = mq[plane + o];
+        ngy = mq[2 * plane + o];
       }
     }
     T bnew = T(0);
     if (use_b && r >= y0) bnew = bb_[(long long)r * pitch + fc];  // b(i, r-1)
 
-    const MT kap_r = MET ? (MT)mk_r : (EXACT ? (MT)kappa_exact(M, i, r) : MT(1));
+    const MT kap_r = MET ? mk_r : kappa_exact(M, i, r);
     const MT gy_r = MET ? (MT)mgy_r : gam_eta_col;
     T qx[3] = {q[0], q[1], q[2]};
 
     // ------------------------------------------------------ xi sweep
     if (r >= 1 && r <= ny) {
       CellQ<T> c;
-      if constexpr (EXACT) c = cell_exact<1>(q[0], q[1], q[2], P);
-      else {
-        if (a.wetdry) c = cell_fast<1, true>(q[0], q[1], q[2], g, dry);
-        else c = cell_fast<1, false>(q[0], q[1], q[2], g, dry);
-      }
+      c = cell_exact<1>(q[0], q[1], q[2], P);
       bad |= (c.bad && check_cell) ? 1 : 0;
       T qL[3] = {shr1(q[0]), shr1(q[1]), shr1(q[2])};
       CellQ<T> cL;
@@ -150,15 +140,14 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
       cL.f1 = shr1(c.f1);
       cL.f2 = shr1(c.f2);
       Waves<T> w;
-      if constexpr (EXACT) w = hll_exact(qL, cL, q, c, P.eps);
-      else w = hll_fast(qL, cL, q, c, eps);
+      w = hll_exact(qL, cL, q, c, P.eps);
       smx = (xface && w.s2 > smx) ? w.s2 : smx;
       T w1R[3], F[3] = {T(0), T(0), T(0)};
 #pragma unroll
       for (int k = 0; k < 3; ++k) w1R[k] = shl1(w.w1[k]);
       // metric neighbours (all lane shifts in wave-uniform control flow)
-      const MT kap_l = (EXACT || MET) ? shr1(kap_r) : MT(1);   // kappa(i-1, r)
-      const MT gx_l = MET ? (MT)mgx_r : (EXACT ? (MT)gamma_xi_exact(M, r) : MT(1));
+      const MT kap_l = shr1(kap_r);                             // kappa(i-1, r)
+      const MT gx_l = MET ? mgx_r : gamma_xi_exact(M, r);
       const MT gx_r = MET ? shl1(gx_l) : gx_l;                  // gamma_xi(i+1, r)
       if (a.ho) {
         T w2L[3];
@@ -166,29 +155,16 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
         for (int k = 0; k < 3; ++k) w2L[k] = shr1(w.w2[k]);
         T p1 = T(1), p2 = T(1);
         if (LIM > 0 && xlim) {
-          if constexpr (EXACT) {
-            p1 = limit_exact<LIM>(w.w1, w1R, P.eps);
-            p2 = limit_exact<LIM>(w.w2, w2L, P.eps);
-          } else {
-            p1 = limit_fast<LIM>(w.w1, w1R, eps);
-            p2 = limit_fast<LIM>(w.w2, w2L, eps);
-          }
+          p1 = limit_exact<LIM>(w.w1, w1R, P.eps);
+          p2 = limit_exact<LIM>(w.w2, w2L, P.eps);
         }
-        if constexpr (EXACT) {
-          const double al = w.s2;
-          const double coef = 0.5 * al * (1.0 - al * dtod_x / (0.5 * (kap_l + kap_r)));
+        const double al = w.s2;
+        const double coef = 0.5 * al * (1.0 - al * dtod_x / (0.5 * (kap_l + kap_r)));
 #pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            double t1 = coef * (p1 * w.w1[k]);
-            double t2 = coef * (p2 * w.w2[k]);
-            F[k] = t1 + t2;
-          }
-        } else {
-          T cfl = w.s2 * dtdx;
-          if (MET) cfl = cfl * __builtin_amdgcn_rcpf(T(0.5) * (kap_l + kap_r));
-          const T coef = (T(0.5) * w.s2) * (T(1) - cfl);
-#pragma unroll
-          for (int k = 0; k < 3; ++k) F[k] = coef * fmaf(p1, w.w1[k], p2 * w.w2[k]);
+        for (int k = 0; k < 3; ++k) {
+          double t1 = coef * (p1 * w.w1[k]);
+          double t2 = coef * (p2 * w.w2[k]);
+          F[k] = t1 + t2;
         }
       }
       const T s2R = shl1(w.s2);
@@ -196,40 +172,20 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) FR[k] = shl1(F[k]);
       if (i >= 1 && i <= nx) {
-        if constexpr (EXACT) {
-          const double fac = dtod_x / kap_r;
+        const double fac = dtod_x / kap_r;
 #pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            double ap = w.s2 * w.w2[k];
-            double am = (-s2R) * w1R[k];
-            double q1 = q[k] - fac * (gx_l * ap + gx_r * am);
-            qx[k] = a.ho ? q1 - dtod_x / kap_r * (FR[k] - F[k]) : q1;
-          }
-        } else if (MET) {
-          const T fac = dtdx * __builtin_amdgcn_rcpf(kap_r);
-          const T sl = gx_l * w.s2, sr = gx_r * s2R;
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            T d = fmaf(sl, w.w2[k], -sr * w1R[k]) + (FR[k] - F[k]);
-            qx[k] = fmaf(-fac, d, q[k]);
-          }
-        } else {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            T d = fmaf(w.s2, w.w2[k], -s2R * w1R[k]) + (FR[k] - F[k]);
-            qx[k] = fmaf(-dtdx, d, q[k]);
-          }
+        for (int k = 0; k < 3; ++k) {
+          double ap = w.s2 * w.w2[k];
+          double am = (-s2R) * w1R[k];
+          double q1 = q[k] - fac * (gx_l * ap + gx_r * am);
+          qx[k] = a.ho ? q1 - dtod_x / kap_r * (FR[k] - F[k]) : q1;
         }
       }
     }
 
     // ------------------------------------------------------ eta sweep
     CellQ<T> cy;
-    if constexpr (EXACT) cy = cell_exact<2>(qx[0], qx[1], qx[2], P);
-    else {
-      if (a.wetdry) cy = cell_fast<2, true>(qx[0], qx[1], qx[2], g, dry);
-      else cy = cell_fast<2, false>(qx[0], qx[1], qx[2], g, dry);
-    }
+    cy = cell_exact<2>(qx[0], qx[1], qx[2], P);
     // the eta sweep reads rows [-1, ny+1] of columns [1, nx]
     bad |= (cy.bad && out_lane && r <= ny + 1) ? 2 : 0;
     Waves<T> Wc;
@@ -237,65 +193,35 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) { Wc.w1[k] = T(0); Wc.w2[k] = T(0); }
     if (r >= y0 - 1) {
-      if constexpr (EXACT) Wc = hll_exact(yq, yc, qx, cy, P.eps);
-      else Wc = hll_fast(yq, yc, qx, cy, eps);
+      Wc = hll_exact(yq, yc, qx, cy, P.eps);
       smy = (out_lane && r >= 0 && r <= ny + 1 && Wc.s2 > smy) ? Wc.s2 : smy;
     }
     T Fa[3] = {T(0), T(0), T(0)};
     if (r >= y0 + 1 && a.ho) {  // correction flux at interface r-1
       T p1 = T(1), p2 = T(1);
       if (LIM > 0 && r - 1 >= 1 && r - 1 <= ny) {
-        if constexpr (EXACT) {
-          p1 = limit_exact<LIM>(Wa.w1, Wc.w1, P.eps);
-          p2 = limit_exact<LIM>(Wa.w2, Wb.w2, P.eps);
-        } else {
-          p1 = limit_fast<LIM>(Wa.w1, Wc.w1, eps);
-          p2 = limit_fast<LIM>(Wa.w2, Wb.w2, eps);
-        }
+        p1 = limit_exact<LIM>(Wa.w1, Wc.w1, P.eps);
+        p2 = limit_exact<LIM>(Wa.w2, Wb.w2, P.eps);
       }
-      if constexpr (EXACT) {
-        const double al = Wa.s2;
-        const double coef = 0.5 * al * (1.0 - al * dtod_y / (0.5 * (kap_pp + kap_p)));
+      const double al = Wa.s2;
+      const double coef = 0.5 * al * (1.0 - al * dtod_y / (0.5 * (kap_pp + kap_p)));
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          double t1 = coef * (p1 * Wa.w1[k]);
-          double t2 = coef * (p2 * Wa.w2[k]);
-          Fa[k] = t1 + t2;
-        }
-      } else {
-        T cfl = Wa.s2 * dtdy;
-        if (MET) cfl = cfl * __builtin_amdgcn_rcpf(T(0.5) * (kap_pp + kap_p));
-        const T coef = (T(0.5) * Wa.s2) * (T(1) - cfl);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) Fa[k] = coef * fmaf(p1, Wa.w1[k], p2 * Wa.w2[k]);
+      for (int k = 0; k < 3; ++k) {
+        double t1 = coef * (p1 * Wa.w1[k]);
+        double t2 = coef * (p2 * Wa.w2[k]);
+        Fa[k] = t1 + t2;
       }
     }
     if (r >= y0 + 2) {  // finish row j = r-2
       const int j = r - 2;
       T qn[3];
-      if constexpr (EXACT) {
-        const double fac = dtod_y / kap_pp;
+      const double fac = dtod_y / kap_pp;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          double ap = Wb.s2 * Wb.w2[k];
-          double am = (-Wa.s2) * Wa.w1[k];
-          double q1 = qpp[k] - fac * (gy_pp * ap + gy_p * am);
-          qn[k] = a.ho ? q1 - dtod_y / kap_pp * (Fa[k] - Fb[k]) : q1;
-        }
-      } else if (MET) {
-        const T fac = dtdy * __builtin_amdgcn_rcpf(kap_pp);
-        const T sl = gy_pp * Wb.s2, sr = gy_p * Wa.s2;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          T d = fmaf(sl, Wb.w2[k], -sr * Wa.w1[k]) + (Fa[k] - Fb[k]);
-          qn[k] = fmaf(-fac, d, qpp[k]);
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          T d = fmaf(Wb.s2, Wb.w2[k], -Wa.s2 * Wa.w1[k]) + (Fa[k] - Fb[k]);
-          qn[k] = fmaf(-dtdy, d, qpp[k]);
-        }
+      for (int k = 0; k < 3; ++k) {
+        double ap = Wb.s2 * Wb.w2[k];
+        double am = (-Wa.s2) * Wa.w1[k];
+        double q1 = qpp[k] - fac * (gy_pp * ap + gy_p * am);
+        qn[k] = a.ho ? q1 - dtod_y / kap_pp * (Fa[k] - Fb[k]) : q1;
       }
       if (do_src) {
         T bxm = shr1(b0), bxp = shl1(b0);
@@ -403,23 +329,23 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<T> a) {
 }
 
 // ------------------------------------------------------------- launcher
-template <typename T, bool EXACT, bool MET>
-static hipError_t launch_lim(int lim, const FusedArgs<T>& a, int blocks, int threads,
+template <bool MET>
+static hipError_t launch_lim(int lim, const FusedArgs<double>& a, int blocks, int threads,
                              hipStream_t s) {
   switch (lim) {
-    case 0: k_fused_step<T, EXACT, 0, MET><<<blocks, threads, 0, s>>>(a); break;
-    case 1: k_fused_step<T, EXACT, 1, MET><<<blocks, threads, 0, s>>>(a); break;
-    case 2: k_fused_step<T, EXACT, 2, MET><<<blocks, threads, 0, s>>>(a); break;
-    case 3: k_fused_step<T, EXACT, 3, MET><<<blocks, threads, 0, s>>>(a); break;
-    default: k_fused_step<T, EXACT, 4, MET><<<blocks, threads, 0, s>>>(a); break;
+    case 0: k_fused_step<0, MET><<<blocks, threads, 0, s>>>(a); break;
+    case 1: k_fused_step<1, MET><<<blocks, threads, 0, s>>>(a); break;
+    case 2: k_fused_step<2, MET><<<blocks, threads, 0, s>>>(a); break;
+    case 3: k_fused_step<3, MET><<<blocks, threads, 0, s>>>(a); break;
+    default: k_fused_step<4, MET><<<blocks, threads, 0, s>>>(a); break;
   }
   return hipGetLastError();
 }
 
 hipError_t launch_fused_f64(int lim, bool met, const FusedArgs<double>& a, int blocks, int threads,
                             hipStream_t s) {
-  return met ? launch_lim<double, true, true>(lim, a, blocks, threads, s)
-             : launch_lim<double, true, false>(lim, a, blocks, threads, s);
+  return met ? launch_lim<true>(lim, a, blocks, threads, s)
+             : launch_lim<false>(lim, a, blocks, threads, s);
 }
 
 }  // namespace pcw
