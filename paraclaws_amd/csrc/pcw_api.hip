@@ -30,6 +30,7 @@ struct pcw_handle {
   KInst* dinst = nullptr;
   InstState* dst = nullptr;
   int4* dtiles = nullptr;
+  TileSlot* dslots = nullptr;     // FP32 completion accumulators, kSlots per instance
   int ntiles = 0;
   void* pool = nullptr;
   long long pool_elems = 0;
@@ -76,6 +77,7 @@ void build_tiles(pcw_handle* h, int ty) {
   h->htiles.clear();
   for (int k = 0; k < h->n; ++k) {
     KInst& I = h->hinst[k];
+    I.tile0 = (int)h->htiles.size();
     std::vector<int> xs, y0s;
     const int nsx = (I.nx + kStripOut - 1) / kStripOut;
     for (int s = 0; s < nsx; ++s) {
@@ -151,7 +153,7 @@ int enqueue_fused(pcw_handle* h) {
     if (h->p.precision == PCW_FP32) {
       FusedArgs<float> a{(float*)h->pool, (const float*)h->bpool, (const float*)h->mpool,
                          h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
-                         wetdry, 0, h->p.grav, h->p.eps, h->p.dry_tol};
+                         wetdry, 0, h->p.grav, h->p.eps, h->p.dry_tol, h->dslots};
       for (int phys = 0; phys <= 1 && e == hipSuccess; ++phys) {
         if (phys ? !any_src : !any_nosrc) continue;
         e = launch_fused_f32(limiter_code(h->p), met, phys, a, blocks, threads, h->stream);
@@ -159,7 +161,7 @@ int enqueue_fused(pcw_handle* h) {
     } else {
       FusedArgs<double> a{(double*)h->pool, (const double*)h->bpool, (const double*)h->mpool,
                           h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
-                          h->p.dry_tol > 0.0, 0, h->p.grav, h->p.eps, h->p.dry_tol};
+                          h->p.dry_tol > 0.0, 0, h->p.grav, h->p.eps, h->p.dry_tol, nullptr};
       e = launch_fused_f64(limiter_code(h->p), met, a, blocks, threads, h->stream);
     }
     if (e != hipSuccess)
@@ -367,6 +369,10 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
   HIPCHK(hipMalloc(&h->dinst, sizeof(KInst) * n_domains));
   HIPCHK(hipMalloc(&h->dst, sizeof(InstState) * n_domains));
   HIPCHK(hipMalloc(&h->dscratch, sizeof(double) * kScratch));
+  if (p->precision == PCW_FP32) {
+    HIPCHK(hipMalloc(&h->dslots, sizeof(TileSlot) * kSlots * n_domains));
+    HIPCHK(hipMemset(h->dslots, 0, sizeof(TileSlot) * kSlots * n_domains));
+  }
   if (p->engine == PCW_ENGINE_STAGED) {
     long long welems = 0;
     for (const auto& I : h->hinst) welems += (long long)W_COUNT * I.plane;
@@ -383,7 +389,7 @@ int pcw_destroy(pcw_handle* h) {
   if (!h) return PCW_ERR_ARG;
   drop_graph(h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* ptrs[] = {h->pool, h->bpool, h->mpool, h->dinst, h->dst, h->dtiles, h->dsnaps, h->dlog,
+  void* ptrs[] = {h->pool, h->bpool, h->mpool, h->dinst, h->dst, h->dtiles, h->dslots, h->dsnaps, h->dlog,
                   h->work, h->dscratch};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);

@@ -17,7 +17,7 @@ constexpr int kStripOut = 60;   // output columns per wave (64 lanes - 2x2 halo)
 struct KInst {
   int nx, ny, pitch, ntiles;
   int bc[4];
-  int source, pad;
+  int source, tile0;   // tile0: index of the instance's first tile in the tile list
   long long plane;     // elements per field plane (pitch * (ny+4))
   long long off[2];    // element offset of the h-plane frame origin (i=-1,j=-1) of buffer 0/1
   long long boff;      // bathymetry frame offset, -1 when absent
@@ -40,6 +40,17 @@ struct TimeCtl {
   const double* snaps;  // snapshot_intervals (device or host pointer)
 };
 
+// FP32 engine: one of kSlots partial accumulators of an instance's attempt
+// (tile t of the instance uses slot t % kSlots), one per 64-byte line so
+// that the tiles' atomics spread over many addresses.
+constexpr int kSlots = 64;
+struct TileSlot {
+  unsigned int mx, my;  // maxima of the sweep speeds (float bits)
+  int bad;              // error bits
+  unsigned int count;   // tiles of this slot done in the running attempt
+  unsigned int pad[12];
+};
+
 // Device-resident per-instance state.
 struct InstState {
   TimeCtl tc;
@@ -49,6 +60,7 @@ struct InstState {
   long long accepted, rejected;
   long long max_accepted;             // stop after this many accepted steps (0 = no limit)
   unsigned int tiles_done;            // completion counter of the running attempt
+  unsigned int smax32_x, smax32_y;    // FP32 engine: running maxima of the sweep speeds (float bits)
   int cur;                            // index of the current buffer
   int status;                         // PCW_STATUS_*
   int error;                          // PCW_ERR_* when status is ERROR

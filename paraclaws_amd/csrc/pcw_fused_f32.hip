@@ -14,8 +14,9 @@
 //     one lane shift of H instead of shifts of s2, w1 and F.
 //   * The eta window's per-row values live in three rotating slots (row r,
 //     r-1, r-2); the row loop is unrolled by three so that the rotation is a
-//     renaming and costs no register moves.  Input rows are prefetched three
-//     rows ahead into three buffers that rotate with the slots.
+//     renaming and costs no register moves.  Input rows are prefetched six
+//     rows ahead (three with metric arrays or source terms) into a ring of
+//     registers that rotates with the slots.
 //   * Rows whose every condition holds (interior rows of interior strips,
 //     away from the tile's first rows and the domain's top and bottom rows)
 //     run a body without any of the range tests; the other rows, and the
@@ -607,10 +608,10 @@ k_step(FusedArgs<float> a) {
     } while (r + 5 <= re);
   }
 
-  // wave maxima; a NaN speed (NaN momentum) is reported like a NaN Jacobian
+  // wave maxima and error bits; the instance's last tile completes the attempt
   const float mx = wave_smax(R.smx), my = wave_smax(R.smy);
-  tile_finish(S, I, cur, (double)mx, (double)my, R.badx != 0ull || mx != mx,
-              R.bady != 0ull || my != my, lane);
+  tile_finish_slots(S, I, a.slots + (size_t)in * kSlots, tid - I.tile0, cur, mx, my,
+                    (R.badx != 0ull ? 1 : 0) | (R.bady != 0ull ? 2 : 0), lane);
 }
 
 template <bool HO, bool MET, bool PHYS>

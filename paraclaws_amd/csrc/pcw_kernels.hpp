@@ -44,6 +44,7 @@ struct FusedArgs {
   int wetdry;          // dry_tol > 0
   int pad;
   double g, eps, dry;
+  TileSlot* slots;     // FP32: kSlots completion accumulators per instance
 };
 
 hipError_t launch_fused_f32(int lim, bool met, bool phys, const FusedArgs<float>& a, int blocks,

@@ -1,5 +1,7 @@
 // pcw_tile.hpp -- end of a fused-step wave: publish the wave's speed maxima
-// and error bits, and let the last tile of an instance finish the attempt.
+// and error bits, and let the last tile of an instance finish the attempt
+// (one level for the FP64 kernel, two levels of accumulators for the FP32
+// kernel).
 #pragma once
 
 #include "pcw_device.hpp"
@@ -7,13 +9,44 @@
 
 namespace pcw {
 
+// Completion of one attempt of an instance from its speed maxima sx, sy and
+// error bits (bit 1: NaN Jacobian or complex eigenvalue in the xi sweep, bit
+// 2: in the eta sweep): record them for the host and, under device control,
+// run the time controller -- the CFL bookkeeping and validity tests of
+// src/solver/MOD_solve.f90:60-175, updateTime of
+// src/time_control/MOD_time_control.f90:153-243 and the buffer flip on accept.
+// cur is the buffer index the attempt read.
+__device__ __forceinline__ void finish_attempt(InstState* S, const KInst& I, int cur, double sx,
+                                               double sy, int badbits) {
+  S->last_smax_x = sx;
+  S->last_smax_y = sy;
+  S->last_bad = badbits;
+  if (!S->controlled) return;  // host control: the host reads last_smax_* and last_bad
+  pcw_step_record rec;
+  int ok;
+  if (finish_checks(S, sx, sy, I.dxi, I.deta, badbits, &rec, &ok)) return;
+  S->cfl_last = rec.cfl;
+  if (ok) {
+    S->cur = cur ^ 1;
+    S->accepted += 1;
+  } else {
+    S->rejected += 1;
+  }
+  if (S->log && S->log_n < S->log_cap) S->log[S->log_n++] = rec;
+  if (S->tc.ended) {
+    S->status = PCW_STATUS_ERROR;
+    S->error = PCW_ERR_CONTROL;
+  } else if (!(S->tc.time < S->tc.time_final) ||
+             (S->max_accepted > 0 && S->accepted >= S->max_accepted)) {
+    S->status = PCW_STATUS_DONE;
+  }
+}
+
 // mx, my: the wave's maxima of the xi / eta interface speeds (already reduced
 // over the wave); bad_x / bad_y: a lane of the wave met a NaN Jacobian or a
 // complex eigenvalue in that sweep.  The instance's last tile (atomic
-// counter) reads the instance maxima, resets the accumulators and, under
-// device control, runs the time controller: the CFL bookkeeping and validity
-// tests of src/solver/MOD_solve.f90:60-175, updateTime of
-// src/time_control/MOD_time_control.f90:153-243, the buffer flip on accept.
+// counter) reads the instance maxima, resets the accumulators and completes
+// the attempt (finish_attempt).
 //
 // Ordering without fences: a device-scope fence on this GPU writes back the
 // issuing XCD's L2, which per wave costs more than the wave's work.  The
@@ -45,28 +78,54 @@ __device__ __forceinline__ void tile_finish(InstState* S, const KInst& I, int cu
   const double sy = __longlong_as_double((long long)atomicExch(&S->smax_y, 0ull));
   const int badbits = atomicExch(&S->bad, 0);
   atomicExch(&S->tiles_done, 0u);
-  S->last_smax_x = sx;
-  S->last_smax_y = sy;
-  S->last_bad = badbits;
-  if (!S->controlled) return;  // host control: the host reads last_smax_* and last_bad
-  pcw_step_record rec;
-  int ok;
-  if (finish_checks(S, sx, sy, I.dxi, I.deta, badbits, &rec, &ok)) return;
-  S->cfl_last = rec.cfl;
-  if (ok) {
-    S->cur = cur ^ 1;
-    S->accepted += 1;
-  } else {
-    S->rejected += 1;
+  finish_attempt(S, I, cur, sx, sy, badbits);
+}
+
+// Two-level completion (FP32 engine, many tiles per instance): tile t of the
+// instance (t = 0 .. ntiles-1) publishes its maxima (float bits; as unsigned
+// integers a NaN wins) and error bits to slot t % kSlots, waits for those
+// atomics and counts itself in the slot.  The slot's last tile moves the
+// slot's results to the instance accumulators the same way and counts the
+// slot; the last slot's last tile completes the attempt.  Every address sees
+// at most ntiles / kSlots + 1 tiles' atomics, instead of all of them.
+__device__ __forceinline__ void tile_finish_slots(InstState* S, const KInst& I, TileSlot* slots,
+                                                  int t, int cur, float mx, float my, int badbits,
+                                                  int lane) {
+  const int nslots = I.ntiles < kSlots ? I.ntiles : kSlots;
+  const int s = t % kSlots;
+  const unsigned per = (unsigned)(I.ntiles / kSlots + (s < I.ntiles % kSlots ? 1 : 0));
+  TileSlot* L = slots + s;
+  unsigned int prev = 0;
+  if (lane == 0) {
+    const unsigned o1 = atomicMax(&L->mx, __builtin_bit_cast(unsigned, mx));
+    const unsigned o2 = atomicMax(&L->my, __builtin_bit_cast(unsigned, my));
+    int o3 = 0;
+    if (badbits) o3 = atomicOr(&L->bad, badbits);
+    asm volatile("" ::"v"(o1), "v"(o2), "v"(o3));  // wait until the three have completed
+    prev = atomicAdd(&L->count, 1u);
   }
-  if (S->log && S->log_n < S->log_cap) S->log[S->log_n++] = rec;
-  if (S->tc.ended) {
-    S->status = PCW_STATUS_ERROR;
-    S->error = PCW_ERR_CONTROL;
-  } else if (!(S->tc.time < S->tc.time_final) ||
-             (S->max_accepted > 0 && S->accepted >= S->max_accepted)) {
-    S->status = PCW_STATUS_DONE;
-  }
+  prev = __shfl(prev, 0, 64);
+  if (prev + 1 != per) return;
+  if (lane != 0) return;
+  // last tile of the slot: move the slot's results to the instance
+  const unsigned smx = atomicExch(&L->mx, 0u);
+  const unsigned smy = atomicExch(&L->my, 0u);
+  const int sbad = atomicExch(&L->bad, 0);
+  atomicExch(&L->count, 0u);
+  const unsigned p1 = atomicMax(&S->smax32_x, smx);
+  const unsigned p2 = atomicMax(&S->smax32_y, smy);
+  int p3 = 0;
+  if (sbad) p3 = atomicOr(&S->bad, sbad);
+  asm volatile("" ::"v"(p1), "v"(p2), "v"(p3));
+  if (atomicAdd(&S->tiles_done, 1u) + 1 != (unsigned)nslots) return;
+  // last slot: complete the attempt
+  const float sx = __builtin_bit_cast(float, atomicExch(&S->smax32_x, 0u));
+  const float sy = __builtin_bit_cast(float, atomicExch(&S->smax32_y, 0u));
+  int bad = atomicExch(&S->bad, 0);
+  atomicExch(&S->tiles_done, 0u);
+  if (sx != sx) bad |= 1;  // a NaN speed (NaN momentum) is reported like a NaN Jacobian
+  if (sy != sy) bad |= 2;
+  finish_attempt(S, I, cur, (double)sx, (double)sy, bad);
 }
 
 }  // namespace pcw
