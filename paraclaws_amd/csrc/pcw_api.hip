@@ -1,8 +1,9 @@
 // pcw_api.hip -- the extern "C" entry points declared in include/pcw.h.
 //
 // Memory: every instance owns two frames (ping-pong) of three SoA planes
-// h, hu, hv of (pitch x (ny+4)) elements, pitch = nx+4 rounded up to 64
-// elements, all in one HBM pool; descriptors (KInst) and the per-instance
+// h, hu, hv of (pitch x (ny+4)) elements, all in one HBM pool (FP64: plain
+// rows, pitch = nx+4 rounded up to 64 elements; FP32: rows of 64-element
+// slots, col_pos in pcw_engine.hpp); descriptors (KInst) and the per-instance
 // controller state (InstState) are device resident.  One HIP stream per
 // handle, default priority.
 #include <hip/hip_runtime.h>
@@ -311,7 +312,13 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
     std::memset(&I, 0, sizeof I);
     I.nx = d.nx;
     I.ny = d.ny;
-    I.pitch = (int)round_up(d.nx + 4, 64);
+    if (p->precision == PCW_FP32) {
+      I.slot_rows = 1;
+      I.nslot = (d.nx + kStripOut - 1) / kStripOut;
+      I.pitch = 64 * (I.nslot + 1);
+    } else {
+      I.pitch = (int)round_up(d.nx + 4, 64);
+    }
     I.plane = (long long)I.pitch * (d.ny + 4);
     I.off[0] = base;
     I.off[1] = base + 3 * I.plane;
@@ -419,7 +426,7 @@ int pcw_set_state(pcw_handle* h, int32_t inst, const double* vars) {
     for (int i = 0; i < nxf; ++i)
       for (int me = 0; me < 3; ++me) {
         double v = vars[me + 3 * ((size_t)i + (size_t)nxf * j)];
-        size_t o = (size_t)me * I.plane + (size_t)j * I.pitch + i;
+        size_t o = (size_t)me * I.plane + (size_t)j * I.pitch + col_pos(I, i - 1);
         if (h->esize == 4) ((float*)buf.data())[o] = (float)v;
         else ((double*)buf.data())[o] = v;
       }
@@ -442,7 +449,7 @@ int pcw_get_state(pcw_handle* h, int32_t inst, double* vars) {
   for (int j = 0; j < nyf; ++j)
     for (int i = 0; i < nxf; ++i)
       for (int me = 0; me < 3; ++me) {
-        size_t o = (size_t)me * I.plane + (size_t)j * I.pitch + i;
+        size_t o = (size_t)me * I.plane + (size_t)j * I.pitch + col_pos(I, i - 1);
         vars[me + 3 * ((size_t)i + (size_t)nxf * j)] =
             h->esize == 4 ? (double)((float*)buf.data())[o] : ((double*)buf.data())[o];
       }
@@ -458,7 +465,7 @@ int pcw_set_bathymetry(pcw_handle* h, int32_t inst, const double* b) {
   for (int j = 0; j < nyf; ++j)
     for (int i = 0; i < nxf; ++i) {
       double v = b[(size_t)i + (size_t)nxf * j];
-      size_t o = (size_t)j * I.pitch + i;
+      size_t o = (size_t)j * I.pitch + col_pos(I, i - 1);
       if (h->esize == 4) ((float*)buf.data())[o] = (float)v;
       else ((double*)buf.data())[o] = v;
     }
@@ -489,7 +496,7 @@ int pcw_set_metrics(pcw_handle* h, int32_t inst, const double* kappa, const doub
     for (int j = 0; j < nyf; ++j)
       for (int i = 0; i < nxf; ++i) {
         double v = src[c][(size_t)i + (size_t)nxf * j];
-        size_t o = (size_t)c * I.plane + (size_t)j * I.pitch + i;
+        size_t o = (size_t)c * I.plane + (size_t)j * I.pitch + col_pos(I, i - 1);
         if (h->esize == 4) ((float*)buf.data())[o] = (float)v;
         else ((double*)buf.data())[o] = v;
       }

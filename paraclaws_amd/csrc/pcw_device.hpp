@@ -1,15 +1,13 @@
 // pcw_device.hpp -- per-cell and per-interface arithmetic of the 2D
 // shallow-water wave-propagation update, for gfx950.
 //
-// Two arithmetic policies:
-//   Exact (double): the reference's FP64 operations in the reference's order
-//     (Jacobian entries of examples/2D_dam_break/cartesian/SMOD_jakobian.f90:44-53,
-//     J*q in BLAS column order, true IEEE divisions, reference-rounded mesh
-//     metrics), so the FP64 mode reproduces the CPU restatement bit for bit.
-//     Every translation unit is compiled with -ffp-contract=off.
-//   Fast (float): the limiter's closed forms with reciprocal-based division
-//     and explicit fmaf, used by the FP32 step kernel (which holds its own
-//     cell and HLL arithmetic).
+// Exact (double) arithmetic: the reference's FP64 operations in the
+// reference's order (Jacobian entries of
+// examples/2D_dam_break/cartesian/SMOD_jakobian.f90:44-53, J*q in BLAS column
+// order, true IEEE divisions, reference-rounded mesh metrics), so the FP64
+// mode reproduces the CPU restatement bit for bit.  Every translation unit is
+// compiled with -ffp-contract=off.  The FP32 kernel's arithmetic lives in
+// pcw_fused_f32.hip.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -160,17 +158,6 @@ __device__ __forceinline__ double phi_exact(double r) {
   return fmax(0.0, fmin(fmin((1.0 + r) / 2.0, 2.0), 2.0 * r));
 }
 
-template <int LIM>
-__device__ __forceinline__ float phi_fast(float r) {
-  if (LIM == 1) return __builtin_amdgcn_fmed3f(r, 0.0f, 1.0f);
-  if (LIM == 2) return fmaxf(fmaxf(0.0f, fminf(1.0f, 2.0f * r)), fminf(2.0f, r));
-  if (LIM == 3) {
-    float a = fabsf(r);
-    return (r + a) * __builtin_amdgcn_rcpf(1.0f + a);
-  }
-  return fmaxf(0.0f, fminf(fminf(fmaf(0.5f, r, 0.5f), 2.0f), 2.0f * r));
-}
-
 // Limiter factor of one wave (MOD_flux_correction.f90:76-120): theta uses
 // the unlimited dot product with the upwind neighbour; a wave whose squared
 // norm is <= ud_precision is left unlimited.
@@ -183,14 +170,6 @@ __device__ __forceinline__ double limit_exact(const double w[3], const double nb
   up = up + nb[2] * w[2];
   if (fabs(wnorm2) <= eps) return 1.0;
   return phi_exact<LIM>(up / wnorm2);
-}
-
-template <int LIM>
-__device__ __forceinline__ float limit_fast(const float w[3], const float nb[3], float eps) {
-  float n = fmaf(w[2], w[2], fmaf(w[1], w[1], w[0] * w[0]));
-  float d = fmaf(nb[2], w[2], fmaf(nb[1], w[1], nb[0] * w[0]));
-  float p = phi_fast<LIM>(d * __builtin_amdgcn_rcpf(n));
-  return n <= eps ? 1.0f : p;
 }
 
 // ---------------------------------------------------------- mesh metrics

@@ -13,11 +13,30 @@ namespace pcw {
 
 constexpr int kStripOut = 60;   // output columns per wave (64 lanes - 2x2 halo)
 
+// Layout of a frame row, columns i = -1 .. nx+2 (ghost, interior, ghost).
+// Plain rows (FP64 engines): column i at element i+1.  Slot rows (FP32
+// engine): the row is a sequence of 64-element slots; slot 0 holds the ghost
+// columns -1, 0 in its last two elements, slot s+1 holds the interior columns
+// 60s+1 .. 60s+60 (kStripOut per wave) and four padding elements, the last
+// slot also the ghost columns nx+1, nx+2 right after column nx.  A wave that
+// finishes the 60 columns of a strip then writes one aligned, whole
+// 256-byte slot per row and plane (partial cache-line writes cost gfx950's
+// memory system about half again the time of whole lines).
+__host__ __device__ inline int col_pos(int slot_rows, int nslot, int i) {
+  if (!slot_rows) return i + 1;
+  if (i <= 0) return 63 + i;
+  int s = (i - 1) / kStripOut;
+  if (s > nslot - 1) s = nslot - 1;
+  return 64 * (s + 1) + (i - 1 - kStripOut * s);
+}
+
 // Read-only description of one instance (domain).
 struct KInst {
   int nx, ny, pitch, ntiles;
   int bc[4];
   int source, tile0;   // tile0: index of the instance's first tile in the tile list
+  int slot_rows;       // 1: frame rows in 64-element slots (FP32 engine), 0: plain rows
+  int nslot;           // slot rows: interior slots, ceil(nx / kStripOut)
   long long plane;     // elements per field plane (pitch * (ny+4))
   long long off[2];    // element offset of the h-plane frame origin (i=-1,j=-1) of buffer 0/1
   long long boff;      // bathymetry frame offset, -1 when absent
@@ -29,6 +48,9 @@ struct KInst {
   // ghost layer (1, 2), from the reference's cell centres
   float gr[8];
 };
+
+// element offset of frame column i within a row
+__host__ __device__ inline int col_pos(const KInst& I, int i) { return col_pos(I.slot_rows, I.nslot, i); }
 
 // time_control fields (MOD_time_control.f90:16-49), FP64.
 struct TimeCtl {

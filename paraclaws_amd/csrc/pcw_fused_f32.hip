@@ -55,6 +55,15 @@ __device__ __forceinline__ float wave_smax(float v) {
   return v;
 }
 
+// Neighbour lanes in xi through the LDS crossbar (ds_bpermute): the shifts
+// do not occupy the VALU, whose DPP forms issue at half the rate of plain
+// FP32 arithmetic on gfx950.  Lane 0 receives lane 63 and vice versa; both
+// are halo lanes, whose results are not stored.
+struct Ctx;
+__device__ __forceinline__ float bperm(int addr, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, v)));
+}
+
 // Buffer descriptors over one field plane of a tile's frame rows y0-1 ..
 // y1+3: row offsets go to the scalar offset, the lane's column to the 32-bit
 // vector offset.  A lane whose vector offset is beyond the descriptor's size
@@ -97,13 +106,17 @@ struct Ctx {
   unsigned lo;          // load offset: this lane's frame column (clamped) * 4
   unsigned so;          // store offset: frame column * 4, out of range off the output lanes
   int lane, i, nx, ny, y0, y1;
+  int bl, br;           // ds_bpermute byte addresses of the left / right neighbour lane
   int src, wetdry, x0, l_nx;
+  int nslot;            // interior slots of a frame row (slot rows)
   int bc[4];
   float gr[8];       // extrapolation ratios of the linear-extrapolation boundary
-  float dtdx, dtdy, hdtdx, hdtdy, eps, g, dry, n2, gdx, gdy, dt;
+  float dtdx, dtdy, eps, g, dry, n2, gdx, gdy, dt;
   bool out_lane, xface, xlim;
   unsigned long long xmask, ymask;  // lanes in the xi sweep's cell range / the output lanes
 };
+__device__ __forceinline__ float lft(const Ctx& C, float v) { return bperm(C.bl, v); }
+__device__ __forceinline__ float rgt(const Ctx& C, float v) { return bperm(C.br, v); }
 
 __device__ __forceinline__ float readlane(float v, int l) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
@@ -172,8 +185,36 @@ __device__ __forceinline__ float hll(const float qL[3], const float fL[3], float
   return live ? s2 : 0.0f;
 }
 
+// Half the limiter function, phi(theta) / 2, of one wave
+// (MOD_flux_correction.f90:76-120; 1 minmod, 2 superbee, 3 van Leer, 4 MC):
+// theta = (upwind wave . wave) / |wave|^2, and a wave with |wave|^2 <= eps is
+// left unlimited (phi = 1).  In halves, the bounds of minmod, superbee and MC
+// are the [0, 1] clamp that gfx950 applies as an output modifier of the
+// instruction producing the value; the caller doubles the flux coefficient
+// instead, which is exact.
+__device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
+template <int LIM>
+__device__ __forceinline__ float half_phi(const float w[3], const float nb[3], float eps) {
+  const float n = fmaf(w[2], w[2], fmaf(w[1], w[1], w[0] * w[0]));
+  const float d = fmaf(nb[2], w[2], fmaf(nb[1], w[1], nb[0] * w[0]));
+  const float t = d * rcp(n);
+  float p;
+  if (LIM == 1) {
+    p = 0.5f * clamp01(t);                                   // max(0, min(1, t)) / 2
+  } else if (LIM == 2) {
+    p = fmaxf(fminf(0.5f, t), clamp01(0.5f * t));            // max(0, min(1, 2t), min(2, t)) / 2
+  } else if (LIM == 3) {
+    const float a = fabsf(t);
+    p = (0.5f * (t + a)) * rcp(1.0f + a);                    // (t + |t|) / (1 + |t|) / 2
+  } else {
+    p = clamp01(fminf(fmaf(0.25f, t, 0.25f), t));            // max(0, min((1+t)/2, 2, 2t)) / 2
+  }
+  return n <= eps ? 0.5f : p;
+}
+
 // correction flux of one interface: coef (p1 w1 + p2 w2) with
-// coef = 0.5 |s| (1 - |s| dt/d / kbar)
+// coef = |s| (1 - |s| dt/d / kbar) and the halved limiter values p1, p2
+// (the reference's 0.5 |s| (1 - |s| dt/d / kbar) phi)
 __device__ __forceinline__ void corr_flux(float coef, float p1, float p2, const float w1[3],
                                           const float w2[3], float F[3]) {
   const float c1 = coef * p1, c2 = coef * p2;
@@ -241,30 +282,30 @@ __device__ __forceinline__ void xi_row(const Ctx& C, int r, const Pre& P, Red& R
       const unsigned long long b = __ballot(!(q[0] > 0.0f));
       R.badx |= GEN ? (b & C.xmask) : b;
     }
-    const float qL[3] = {shr1(q[0]), shr1(q[1]), shr1(q[2])};
-    const float fL[3] = {qL[1], shr1(f1), shr1(f2)};
+    const float qL[3] = {lft(C, q[0]), lft(C, q[1]), lft(C, q[2])};
+    const float fL[3] = {qL[1], lft(C, f1), lft(C, f2)};
     const float fR[3] = {q[1], f1, f2};
-    const float sL = shr1(s);
+    const float sL = lft(C, s);
     float w1[3], w2[3];
     const float s2 = hll(qL, fL, sL, q, fR, s, C.eps, w1, w2);
     R.smx = smax2(R.smx, (!GEN || C.xface) ? s2 : 0.0f);
     float F[3] = {0.0f, 0.0f, 0.0f};
     const float kap_l = MET ? shr1(mk) : 1.0f;
     if (HO) {
-      float p1 = 1.0f, p2 = 1.0f;
+      float p1 = 0.5f, p2 = 0.5f;  // halved limiter values (unlimited: 1 / 2)
       if (LIM > 0) {
-        const float w1R[3] = {shl1(w1[0]), shl1(w1[1]), shl1(w1[2])};
-        const float w2L[3] = {shr1(w2[0]), shr1(w2[1]), shr1(w2[2])};
-        p1 = limit_fast<LIM>(w1, w1R, C.eps);
-        p2 = limit_fast<LIM>(w2, w2L, C.eps);
+        const float w1R[3] = {rgt(C, w1[0]), rgt(C, w1[1]), rgt(C, w1[2])};
+        const float w2L[3] = {lft(C, w2[0]), lft(C, w2[1]), lft(C, w2[2])};
+        p1 = half_phi<LIM>(w1, w1R, C.eps);
+        p2 = half_phi<LIM>(w2, w2L, C.eps);
         if (GEN) {
-          p1 = C.xlim ? p1 : 1.0f;
-          p2 = C.xlim ? p2 : 1.0f;
+          p1 = C.xlim ? p1 : 0.5f;
+          p2 = C.xlim ? p2 : 0.5f;
         }
       }
-      // 0.5 s (1 - s dt/dxi / kbar); kbar = 1 without metric arrays
-      const float coef = MET ? 0.5f * s2 * fmaf(-s2 * C.dtdx, rcp(0.5f * (kap_l + mk)), 1.0f)
-                             : s2 * fmaf(-C.hdtdx, s2, 0.5f);
+      // s (1 - s dt/dxi / kbar); kbar = 1 without metric arrays
+      const float coef = MET ? s2 * fmaf(-s2 * C.dtdx, rcp(0.5f * (kap_l + mk)), 1.0f)
+                             : s2 * fmaf(-C.dtdx, s2, 1.0f);
       corr_flux(coef, p1, p2, w1, w2, F);
     }
     const float sg = MET ? mgx * s2 : s2;
@@ -272,7 +313,7 @@ __device__ __forceinline__ void xi_row(const Ctx& C, int r, const Pre& P, Red& R
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       G[k] = fmaf(sg, w2[k], -F[k]);
-      HR[k] = shl1(fmaf(-sg, w1[k], F[k]));
+      HR[k] = rgt(C, fmaf(-sg, w1[k], F[k]));
     }
     const float fac = MET ? C.dtdx * rcp(mk) : C.dtdx;
 #pragma unroll
@@ -370,13 +411,13 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
                                        float bnew, float fin1[3], float finm[3]) {
   float Fa[3] = {0.0f, 0.0f, 0.0f};
   if (HO) {
-    float p1 = 1.0f, p2 = 1.0f;
+    float p1 = 0.5f, p2 = 0.5f;  // halved limiter values (unlimited: 1 / 2)
     if (LIM > 0 && (!GEN || (r - 1 >= 1 && r - 1 <= C.ny))) {
-      p1 = limit_fast<LIM>(A.w1, N.w1, C.eps);
-      p2 = limit_fast<LIM>(A.w2, B.w2, C.eps);
+      p1 = half_phi<LIM>(A.w1, N.w1, C.eps);
+      p2 = half_phi<LIM>(A.w2, B.w2, C.eps);
     }
-    const float coef = MET ? 0.5f * A.s2 * fmaf(-A.s2 * C.dtdy, rcp(0.5f * (B.kap + A.kap)), 1.0f)
-                           : A.s2 * fmaf(-C.hdtdy, A.s2, 0.5f);
+    const float coef = MET ? A.s2 * fmaf(-A.s2 * C.dtdy, rcp(0.5f * (B.kap + A.kap)), 1.0f)
+                           : A.s2 * fmaf(-C.dtdy, A.s2, 1.0f);
     corr_flux(coef, p1, p2, A.w1, A.w2, Fa);
   }
   const float sg = MET ? A.gy * A.s2 : A.s2;
@@ -415,7 +456,7 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
 #pragma unroll
           for (int k = 0; k < 3; ++k)
             bstore(ghost(C.bc[0], nr[k], fr[k], kk, k == 1, kk == 1 ? C.gr[0] : C.gr[1]), C.qo[k],
-                   4u * C.lane, sj);
+                   4u * (unsigned)col_pos(1, C.nslot, C.lane - 1), sj);
         }
       }
       if (C.l_nx >= 3 && C.l_nx <= 61) {
@@ -427,7 +468,7 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
 #pragma unroll
           for (int k = 0; k < 3; ++k)
             bstore(ghost(C.bc[1], nr[k], fr[k], kk, k == 1, kk == 1 ? C.gr[2] : C.gr[3]), C.qo[k],
-                   4u * (C.nx + 1 + kk), sj);
+                   4u * (unsigned)col_pos(1, C.nslot, C.nx + kk), sj);
         }
       }
       // eta-boundary ghosts
@@ -492,13 +533,27 @@ k_step(FusedArgs<float> a) {
   C.y0 = t.z;
   C.y1 = t.w;
   C.lane = lane;
+  C.bl = ((lane + 63) & 63) * 4;
+  C.br = ((lane + 1) & 63) * 4;
   C.i = C.x0 - 2 + lane;                              // this lane's column (1-based interior)
   C.l_nx = C.nx - C.x0 + 2;                           // lane of column nx
   C.out_lane = lane >= 2 && lane <= 61 && C.i >= 1 && C.i <= C.nx;
   C.p4 = I.pitch * 4;
   C.rbase = C.y0 - 1;
-  C.lo = 4u * (unsigned)(C.i + 1 < C.nx + 3 ? C.i + 1 : C.nx + 3);
-  C.so = C.out_lane ? 4u * (unsigned)(C.i + 1) : kOff;
+  // frame rows are in 64-element slots (col_pos); the host lays out every
+  // FP32 instance that way
+  C.nslot = I.nslot;
+  C.lo = 4u * (unsigned)col_pos(1, C.nslot, C.i < C.nx + 2 ? C.i : C.nx + 2);
+  if ((C.x0 - 1) % kStripOut == 0 && C.x0 + kStripOut - 1 <= C.nx) {
+    // the strip fills slot s+1: all 64 lanes store, the four halo lanes
+    // into the slot's padding, so that every row store is one aligned
+    // 256-byte slot
+    const int s = (C.x0 - 1) / kStripOut;
+    C.so = 4u * (unsigned)(64 * (s + 1) + ((lane - 2) & 63));
+  } else {
+    // the strip shifted to end at column nx: output lanes only
+    C.so = C.out_lane ? 4u * (unsigned)col_pos(1, C.nslot, C.i) : kOff;
+  }
   {
     const long long o = (long long)C.rbase * I.pitch;  // descriptor origin within a plane
     const int bytes = (C.y1 - C.y0 + 5) * C.p4;
@@ -524,8 +579,6 @@ k_step(FusedArgs<float> a) {
   for (int k = 0; k < 8; ++k) C.gr[k] = I.gr[k];
   C.dtdx = (float)(dt / I.dxi);
   C.dtdy = (float)(dt / I.deta);
-  C.hdtdx = 0.5f * C.dtdx;
-  C.hdtdy = 0.5f * C.dtdy;
   C.eps = (float)a.eps;
   C.g = (float)a.g;
   C.dry = (float)a.dry;

@@ -20,6 +20,11 @@ namespace pcw {
 __device__ __forceinline__ long long fidx(int pitch, int i, int j) {
   return (long long)(j + 1) * pitch + (i + 1);
 }
+// element of frame cell (i, j) in either row layout (the kernels shared with
+// the FP32 engine: ghosts, source, totals)
+__device__ __forceinline__ long long fidx(const KInst& I, int i, int j) {
+  return (long long)(j + 1) * I.pitch + col_pos(I, i);
+}
 
 // metrics of cell (i, j): metric planes of a mapped grid, else the identity map's
 __device__ __forceinline__ double st_kappa(const StagedArgs& a, const KInst& I, int i, int j) {
@@ -223,7 +228,6 @@ __global__ void k_ghosts(T* pool, const KInst* inst, InstState* st, int buf_sel)
   const int b = buf_sel < 0 ? S->cur : (S->cur ^ 1);
   T* q = pool + I.off[b];
   const long long pl = I.plane;
-  const int p = I.pitch;
   int side, pos, kk;
   if (t < 2 * nlr) { side = t / nlr; pos = (t % nlr) / 2 + 1; kk = (t % 2) + 1; }
   else { int u = t - 2 * nlr; side = 2 + u / nbt; pos = (u % nbt) / 2 + 1; kk = (u % 2) + 1; }
@@ -232,19 +236,19 @@ __global__ void k_ghosts(T* pool, const KInst* inst, InstState* st, int buf_sel)
   double xn, xf, xg;
   int normal;
   if (side == 0) {
-    kn = fidx(p, 1, pos); kf = fidx(p, 2, pos); kg = fidx(p, 1 - kk, pos);
+    kn = fidx(I, 1, pos); kf = fidx(I, 2, pos); kg = fidx(I, 1 - kk, pos);
     xn = centre(M.xi_min, M.dxi, 1); xf = centre(M.xi_min, M.dxi, 2); xg = centre(M.xi_min, M.dxi, 1 - kk);
     normal = 1;
   } else if (side == 1) {
-    kn = fidx(p, nx, pos); kf = fidx(p, nx - 1, pos); kg = fidx(p, nx + kk, pos);
+    kn = fidx(I, nx, pos); kf = fidx(I, nx - 1, pos); kg = fidx(I, nx + kk, pos);
     xn = centre(M.xi_min, M.dxi, nx); xf = centre(M.xi_min, M.dxi, nx - 1); xg = centre(M.xi_min, M.dxi, nx + kk);
     normal = 1;
   } else if (side == 2) {
-    kn = fidx(p, pos, 1); kf = fidx(p, pos, 2); kg = fidx(p, pos, 1 - kk);
+    kn = fidx(I, pos, 1); kf = fidx(I, pos, 2); kg = fidx(I, pos, 1 - kk);
     xn = centre(M.eta_min, M.deta, 1); xf = centre(M.eta_min, M.deta, 2); xg = centre(M.eta_min, M.deta, 1 - kk);
     normal = 2;
   } else {
-    kn = fidx(p, pos, ny); kf = fidx(p, pos, ny - 1); kg = fidx(p, pos, ny + kk);
+    kn = fidx(I, pos, ny); kf = fidx(I, pos, ny - 1); kg = fidx(I, pos, ny + kk);
     xn = centre(M.eta_min, M.deta, ny); xf = centre(M.eta_min, M.deta, ny - 1); xg = centre(M.eta_min, M.deta, ny + kk);
     normal = 2;
   }
@@ -265,12 +269,12 @@ __global__ void k_source(T* pool, const T* bpool, const KInst* inst, InstState* 
   if (i > I.nx || j > I.ny) return;
   if (I.source == PCW_SRC_NONE && !wetdry) return;
   T* q = pool + I.off[S->cur];
-  const long long pl = I.plane, k = fidx(I.pitch, i, j);
+  const long long pl = I.plane, k = fidx(I, i, j);
   T bxm = 0, bxp = 0, bym = 0, byp = 0;
   if (I.boff >= 0) {
     const T* b = bpool + I.boff;
-    bxm = b[fidx(I.pitch, i - 1, j)]; bxp = b[fidx(I.pitch, i + 1, j)];
-    bym = b[fidx(I.pitch, i, j - 1)]; byp = b[fidx(I.pitch, i, j + 1)];
+    bxm = b[fidx(I, i - 1, j)]; bxp = b[fidx(I, i + 1, j)];
+    bym = b[fidx(I, i, j - 1)]; byp = b[fidx(I, i, j + 1)];
   }
   T h = q[k], hu = q[pl + k], hv = q[2 * pl + k];
   apply_source<T>(h, hu, hv, I.source, (T)dry, (T)g, (T)I.dxi, (T)I.deta, (T)S->tc.dt,
@@ -346,7 +350,7 @@ __global__ void k_totals_partial(const T* pool, const KInst* inst, const InstSta
   double s[3] = {0.0, 0.0, 0.0};
   for (int j = 1 + blockIdx.x; j <= I.ny; j += gridDim.x)
     for (int i = 1 + threadIdx.x; i <= I.nx; i += blockDim.x) {
-      long long k = fidx(I.pitch, i, j);
+      long long k = fidx(I, i, j);
       s[0] += (double)q[k];
       s[1] += (double)q[pl + k];
       s[2] += (double)q[2 * pl + k];
