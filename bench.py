@@ -63,14 +63,15 @@ TIME_KW = dict(t_final=0.5, dt0=1e-10, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10
 
 
 def run_engine(abi, ic, n, precision, engine, steps, warmup, spg, autotune, log_steps=0,
-               repeat=1):
+               repeat=1, tiles=None):
     """Dam break n^2 from the initial state: `warmup` untimed accepted steps,
     then `repeat` timed runs of exactly `steps` accepted steps, each from the
     initial state.  The returned record is the run with the median wall time
     (walls of all runs in `walls_s`); the engine holds that run's state."""
     dom = abi.Domain(n, n, bc=(4, 4, 4, 4))
     q0 = ic.dam_break(n, n)
-    e = abi.Engine([dom], precision=precision, engine=engine, limiter=4)
+    e = abi.Engine([dom], precision=precision, engine=engine, limiter=4,
+                   **(dict(tile_rows=tiles[0], waves_per_block=tiles[1]) if tiles else {}))
 
     def reset():
         e.set_state(0, q0)
@@ -78,8 +79,8 @@ def run_engine(abi, ic, n, precision, engine, steps, warmup, spg, autotune, log_
         e.time_init(**TIME_KW)
 
     reset()
-    tuned = None
-    if autotune:
+    tuned = tuple(tiles) if tiles else None
+    if autotune and not tiles:
         tuned = e.autotune(steps=3)
         log(f"  autotune -> tile_rows={tuned[0]} waves_per_block={tuned[1]}")
     if warmup:
@@ -285,6 +286,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tiles", type=int, nargs=2, metavar=("ROWS", "WAVES"),
+                    help="FP32 8192^2 leg: tile height and waves per block instead of the "
+                         "autotuned ones (profiler runs, whose timing would mislead the autotuner)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-physics", action="store_true",
                     help="skip the bathymetry (8192^2) and friction-batch legs")
@@ -317,7 +321,8 @@ def main():
     log(f"FP32 fused engine, dam break {n}^2, {args.steps} accepted steps, "
         f"median of {args.repeat} runs")
     e32, r32 = run_engine(abi, ic, n, abi.FP32, abi.FUSED, args.steps, args.warmup,
-                          args.steps_per_graph, not args.no_autotune, repeat=args.repeat)
+                          args.steps_per_graph, not args.no_autotune, repeat=args.repeat,
+                          tiles=args.tiles)
     value = r32["accepted"] * cells / r32["wall_s"]
     log(f"  wall {r32['wall_s'] * 1e3:.2f} ms  {value:.4e} cell-updates/s  "
         f"kernel avg {r32['avg_step_ms']:.4f} ms  rejected {r32['rejected']}")
