@@ -128,3 +128,37 @@ def test_gpu_mixed_batch_mapped_and_plain(oracle):
                 e1.run(steps_per_graph=8)
                 solo = e1.get_state(0)
             assert np.array_equal(both[idx], solo), (prec, idx)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mapping", [STRETCH, CURVED], ids=["stretch", "curved"])
+def test_gpu_fp32_mapped_close(oracle, mapping):
+    """FP32 fused step with metric arrays (FP32 frames in 64-element slot rows,
+    metric planes in the same rows) against the FP64 oracle: interior and
+    ghost cells within 4e-6 of the field scale, CFL within 1e-6 relative.
+    nx = 175 gives two strips that fill their slots and one shifted to end at
+    column nx."""
+    from paraclaws_amd import abi
+
+    nx, ny = 175, 66
+    cfg, (k, gx, ge, _, _) = _cfg(oracle, nx, ny, mapping, limiter=4, bc=(4, 3, 2, 4))
+    q = rough_state(nx, ny, seed=9)
+    oracle.fill_ghosts(cfg, q)
+    dt = 0.5 * stable_dt(q, nx, ny)
+    ref = q.copy()
+    tc = oracle.TimeSettings(dt0=dt, shrink=1e-9).tc()
+    assert oracle.solve(cfg, tc, ref) == 1
+    oracle.fill_ghosts(cfg, ref)
+    print("gpu fp32 mapped", flush=True)
+    with abi.Engine([abi.Domain(nx, ny, bc=cfg.bc)], precision=abi.FP32, limiter=4) as e:
+        e.set_state(0, q)
+        e.set_metrics(0, k, gx, ge)
+        smax = e.step(dt)[0]
+        e.accept()
+        got = e.get_state(0)
+    m = sides_mask(nx, ny)
+    for c in range(3):
+        d = np.abs(got[..., c][m] - ref[..., c][m])
+        assert d.max() <= 4e-6 * max(1.0, np.abs(ref[..., c][m]).max()), (c, d.max())
+    cfl = max(smax[0] * dt / (2 / nx), smax[1] * dt / (2 / ny))
+    assert abs(cfl - tc.CFL) <= 1e-6 * tc.CFL
