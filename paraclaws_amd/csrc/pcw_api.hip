@@ -74,6 +74,50 @@ int fail(pcw_handle* h, int code, const std::string& msg) {
 
 long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
 
+// A tile that touches a domain boundary (the first strip, the strip ending at
+// column nx, the first two or last two rows) runs the kernels' general body
+// and takes longer than an interior tile.
+bool boundary_tile(const KInst& I, const int4& t) {
+  return t.y == 1 || t.y + kStripOut - 1 >= I.nx || t.z < 3 || t.w + 2 > I.ny;
+}
+
+// The fused kernels give XCD x the blocks x*nb/8 .. (x+1)*nb/8-1 of the grid
+// (nb blocks of wpb tiles), in dispatch order.  An instance's boundary tiles
+// are spread over the XCD ranges its tiles cover, in proportion to each
+// range's share of them, and each range starts with its share: boundary tiles
+// then start in the first round of resident waves on every XCD instead of
+// finishing last on some.  Interior tiles keep their order, and every
+// instance keeps its contiguous range [tile0, tile0+ntiles).
+void order_tiles(pcw_handle* h) {
+  const long long T = (long long)h->htiles.size();
+  if (T == 0) return;
+  const long long nb = round_up((T + h->wpb - 1) / h->wpb, 8);
+  const long long span = nb / 8 * h->wpb;
+  long long i0 = 0;
+  while (i0 < T) {
+    const int k = h->htiles[i0].x;
+    long long i1 = i0;
+    while (i1 < T && h->htiles[i1].x == k) ++i1;
+    const KInst& I = h->hinst[k];
+    std::vector<int4> bnd, inner;
+    for (long long i = i0; i < i1; ++i)
+      (boundary_tile(I, h->htiles[i]) ? bnd : inner).push_back(h->htiles[i]);
+    const long long n = i1 - i0, nbnd = (long long)bnd.size();
+    long long out = i0, bi = 0, ii = 0;
+    for (long long a = i0 / span * span; a < i1; a += span) {
+      const long long hi = std::min(a + span, i1);
+      // boundary tiles due up to the end of this range, in proportion
+      const long long due = (hi == i1) ? nbnd : nbnd * (hi - i0) / n;
+      for (; bi < due && out < hi; ++bi) h->htiles[out++] = bnd[bi];
+      while (out < hi) {
+        if (ii < (long long)inner.size()) h->htiles[out++] = inner[ii++];
+        else h->htiles[out++] = bnd[bi++];
+      }
+    }
+    i0 = i1;
+  }
+}
+
 void build_tiles(pcw_handle* h, int ty) {
   h->htiles.clear();
   for (int k = 0; k < h->n; ++k) {
@@ -100,6 +144,7 @@ void build_tiles(pcw_handle* h, int ty) {
     I.ntiles = count;
   }
   h->ntiles = (int)h->htiles.size();
+  order_tiles(h);
 }
 
 int upload_tiles(pcw_handle* h) {
