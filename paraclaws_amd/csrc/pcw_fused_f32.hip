@@ -1,11 +1,14 @@
 // pcw_fused_f32.hip -- the FP32 fused step: one kernel per time step (xi
 // sweep, eta sweep, source term, physical ghost fill) for any number of
-// instances, built for VALU throughput on gfx950.
+// instances, built for gfx950's memory system and VALU.
 //
-// Mapping (wave64, no LDS): as in the FP64 fused kernel, a wave owns a strip
-// of 60 output columns plus two halo lanes on each side and marches down the
-// rows of its tile; neighbour states in xi come from DPP lane shifts, and the
-// eta sweep runs in a register window two rows behind the xi sweep.
+// Mapping (wave64): as in the FP64 fused kernel, a wave owns a strip of 60
+// output columns plus two halo lanes on each side and marches down the rows
+// of its tile; neighbour states in xi come from lane shifts through the LDS
+// crossbar (ds_bpermute), and the eta sweep runs in a register window two
+// rows behind the xi sweep.  Frame rows are in 64-element slots (col_pos in
+// pcw_engine.hpp): a wave's 60 output columns fill one slot, so each finished
+// row is one aligned 256-byte store per plane.
 //
 // What differs from the FP64 kernel:
 //   * Fluctuations are combined per interface: G = A+dQ - F goes to the
