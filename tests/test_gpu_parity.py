@@ -291,6 +291,49 @@ def test_batch_equals_solo_and_oracle(oracle):
                 assert np.array_equal(batch[k][m], ref[m]), k
 
 
+def test_batch_tile_order_equals_solo(oracle):
+    """Larger uneven instances with short tiles, so that the tile list spans
+    every XCD range and holds interior and boundary tiles (the host reorders
+    them per range): each instance of the batch equals the instance run alone,
+    bit for bit in FP32 and FP64, and the FP64 batch equals the oracle."""
+    abi = _abi()
+    O = oracle
+    sizes = [(300, 250, 2.2), (250, 301, 3.0), (181, 97, 1.6)]
+    doms, qs, cfgs = [], [], []
+    for nx, ny, r in sizes:
+        cfg = O.Config(nx=nx, ny=ny, limiter=4, bc=(4, 3, 2, 4))
+        q = O.dam_break(cfg, h_in=r)
+        O.fill_ghosts(cfg, q)
+        doms.append(abi.Domain(nx, ny, bc=cfg.bc))
+        qs.append(q)
+        cfgs.append(cfg)
+    tkw = dict(t_final=0.004, dt0=1e-4, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10, growth=1.2,
+               shrink=0.5, n_snapshots=1)
+    for prec in (abi.FP64, abi.FP32):
+        _say(f"batch tile order precision={prec}")
+        with abi.Engine(doms, precision=prec, limiter=4, tile_rows=16, waves_per_block=4) as e:
+            for k, q in enumerate(qs):
+                e.set_state(k, q)
+            e.time_init(**tkw)
+            e.run(steps_per_graph=8)
+            batch = [e.get_state(k) for k in range(len(doms))]
+            stats = [e.status(k) for k in range(len(doms))]
+        for k, d in enumerate(doms):
+            assert stats[k].status == abi.STATUS_DONE
+            with abi.Engine([d], precision=prec, limiter=4, tile_rows=37, waves_per_block=2) as e1:
+                e1.set_state(0, qs[k])
+                e1.time_init(**tkw)
+                e1.run(steps_per_graph=8)
+                solo = e1.get_state(0)
+            assert np.array_equal(batch[k], solo), (prec, k)
+        if prec == abi.FP64:
+            ref = qs[2].copy()
+            ts = O.TimeSettings(t_final=tkw["t_final"], dt0=tkw["dt0"], n_snap=1)
+            O.run(cfgs[2], ts, ref)
+            m = sides_mask(doms[2].nx, doms[2].ny)
+            assert np.array_equal(batch[2][m], ref[m])
+
+
 def test_totals(oracle):
     abi = _abi()
     nx, ny = 300, 170
