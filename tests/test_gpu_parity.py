@@ -83,6 +83,10 @@ FP32_CASES = CASES + [
     (250, 301, 4, True, (4, 3, 2, 4), "rough", 64),
     (247, 190, 2, True, (3, 2, 4, 3), "smooth", 16),
     (181, 97, 1, True, (2, 4, 3, 2), "rough", 37),
+    # nx a multiple of 60: the last strip fills its slot and its right ghost
+    # columns sit in that slot's padding
+    (120, 45, 4, True, (3, 3, 4, 2), "rough", 16),
+    (60, 70, 2, True, (2, 4, 3, 3), "smooth"),
 ]
 
 
