@@ -62,7 +62,6 @@ __device__ __forceinline__ float wave_smax(float v) {
 // do not occupy the VALU, whose DPP forms issue at half the rate of plain
 // FP32 arithmetic on gfx950.  Lane 0 receives lane 63 and vice versa; both
 // are halo lanes, whose results are not stored.
-struct Ctx;
 __device__ __forceinline__ float bperm(int addr, float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, v)));
 }
@@ -106,8 +105,8 @@ struct Ctx {
   bool has_b;
   int p4;               // row pitch in bytes
   int rbase;            // frame row at the descriptors' origin (y0 - 1)
-  unsigned lo;          // load offset: this lane's frame column (clamped) * 4
-  unsigned so;          // store offset: frame column * 4, out of range off the output lanes
+  unsigned lo;          // load offset: this lane's position in the slot row (clamped) * 4
+  unsigned so;          // store offset: position * 4 (whole slot, or out of range off the output lanes)
   int lane, i, nx, ny, y0, y1;
   int bl, br;           // ds_bpermute byte addresses of the left / right neighbour lane
   int src, wetdry, x0, l_nx;
