@@ -213,6 +213,12 @@ int enqueue_fused(pcw_handle* h) {
     if (e != hipSuccess)
       return fail(h, PCW_ERR_HIP, std::string("fused launch: ") + hipGetErrorString(e));
   }
+  if (h->p.precision == PCW_FP64) {  // ghost cells of the new FP64 state
+    hipError_t e = launch_ghosts<double>((double*)h->pool, h->dinst, h->dst, h->n, h->max_side,
+                                         kGhostsAfterFused, h->stream);
+    if (e != hipSuccess)
+      return fail(h, PCW_ERR_HIP, std::string("ghost launch: ") + hipGetErrorString(e));
+  }
   return PCW_OK;
 }
 
@@ -249,7 +255,7 @@ int enqueue_source_ghosts(pcw_handle* h) {
                          h->max_ny, h->p.dry_tol > 0.0, h->p.grav, h->p.dry_tol, h->stream);
     if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
   }
-  e = launch_ghosts<T>((T*)h->pool, h->dinst, h->dst, h->n, h->max_side, -1, h->stream);
+  e = launch_ghosts<T>((T*)h->pool, h->dinst, h->dst, h->n, h->max_side, kGhostsCur, h->stream);
   if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
   return PCW_OK;
 }
@@ -407,7 +413,9 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
   *out = h;
   h->pool_elems = base;
   h->bpool_elems = bbase;
-  h->tile_rows = p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 32);
+  // default tile heights: FP32 64 rows (memory-bound; the autotuner refines it), FP64
+  // 128 rows (arithmetic-bound: fewer recomputed halo rows per tile)
+  h->tile_rows = p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128);
   h->wpb = (p->waves_per_block >= 1 && p->waves_per_block <= 4) ? p->waves_per_block : 2;
   HIPCHK(hipStreamCreate(&h->stream));
   HIPCHK(hipEventCreate(&h->ev0));
@@ -648,8 +656,8 @@ int pcw_source(pcw_handle* h, const double* dt) {
 int pcw_fill_ghosts(pcw_handle* h) {
   if (!h) return PCW_ERR_ARG;
   hipError_t e = h->esize == 4
-                     ? launch_ghosts<float>((float*)h->pool, h->dinst, h->dst, h->n, h->max_side, -1, h->stream)
-                     : launch_ghosts<double>((double*)h->pool, h->dinst, h->dst, h->n, h->max_side, -1, h->stream);
+                     ? launch_ghosts<float>((float*)h->pool, h->dinst, h->dst, h->n, h->max_side, kGhostsCur, h->stream)
+                     : launch_ghosts<double>((double*)h->pool, h->dinst, h->dst, h->n, h->max_side, kGhostsCur, h->stream);
   if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
   HIPCHK(hipStreamSynchronize(h->stream));
   return PCW_OK;
@@ -681,6 +689,7 @@ int pcw_time_init(pcw_handle* h, const pcw_time* t) {
     S.bad = 0;
     S.controlled = 1;
     S.last_ok = 1;
+    S.ghosts_due = 0;
     S.log_n = 0;
   }
   return push_states(h);

@@ -90,6 +90,7 @@ struct InstState {
   int last_bad;                       // bits of the last completed attempt
   int controlled;                     // 1: the last tile runs the controller
   int last_ok;                        // staged engine: 1 when the last attempt was kept
+  int ghosts_due;                     // FP64 fused engine: 1 when the attempt just completed was accepted
   int log_n, log_cap;
   pcw_step_record* log;
 };

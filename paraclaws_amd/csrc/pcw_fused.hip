@@ -1,5 +1,7 @@
-// pcw_fused.hip -- one kernel per time step: xi sweep, eta sweep, source
-// term and physical ghost fill, for any number of instances at once.
+// pcw_fused.hip -- FP64 step kernel: xi sweep, eta sweep and source term of
+// one time step, for any number of instances at once.  The physical ghost
+// fill of the new state follows in a small kernel of its own (k_ghosts), so
+// that this one stays within 168 VGPRs (three waves per SIMD).
 //
 // Mapping (gfx950, wave64, no LDS): a wave owns a strip of 60 output
 // columns plus two halo lanes on each side and marches down the rows of its
@@ -14,8 +16,7 @@
 // Reference stages fused here (src/solver/MOD_solve.f90:23-175 and callees):
 // wave_decomposition/HLL, split_flux, integrate_flux_difference, limiter,
 // second_order_flux_correction, integrator_flux_correction_increment, the
-// CFL max (as max wave speed), integrate_source_term and
-// apply_physical_boundaries.
+// CFL max (as max wave speed) and integrate_source_term.
 #include "pcw_device.hpp"
 #include "pcw_engine.hpp"
 #include "pcw_kernels.hpp"
@@ -38,8 +39,12 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   const int in = __builtin_amdgcn_readfirstlane(t.x);
   const KInst& I = a.inst[in];
   InstState* S = a.st + in;
-  if (S->status != PCW_STATUS_RUNNING) return;
   if (MET != (I.moff >= 0)) return;  // instances with and without metric arrays run in separate launches
+  if (S->status != PCW_STATUS_RUNNING) {
+    // no attempt this launch: the ghost fill that follows must not run
+    if (tid == I.tile0 && lane == 0) S->ghosts_due = 0;
+    return;
+  }
 
   const int nx = I.nx, ny = I.ny, pitch = I.pitch;
   const long long plane = I.plane;
@@ -72,10 +77,6 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   const T n2 = (T)(I.manning * I.manning);
   const T* __restrict__ bb_ = (I.boff >= 0) ? a.bpool + I.boff : nullptr;
   const bool use_b = bb_ && (src == PCW_SRC_BATHY || src == PCW_SRC_BATHY_FRICTION);
-  const int bc_l = I.bc[0], bc_r = I.bc[1], bc_b = I.bc[2], bc_t = I.bc[3];
-  const int l_nx = nx - x0 + 2;                // lane of column nx
-  const bool do_left = x0 == 1;
-  const bool do_right = l_nx >= 3 && l_nx <= 61;
   using MT = double;
   const MT gam_eta_col = gamma_eta_exact(M, i);
 
@@ -93,8 +94,6 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   T qpp[3] = {T(0), T(0), T(0)};      // row r-2 (xi-updated)
   MT kap_p = MT(1), kap_pp = MT(1);   // kappa(i, r-1), kappa(i, r-2)
   MT gy_p = MT(1), gy_pp = MT(1);     // gamma_eta(i, r-1), gamma_eta(i, r-2) (metric arrays)
-  T fin1[3] = {T(0), T(0), T(0)};     // first finished row (bottom ghosts)
-  T finm[3] = {T(0), T(0), T(0)};     // previous finished row (top ghosts)
   T bm = T(0), b0 = T(0);             // bathymetry rows r-3, r-2 of this column
 
   // prefetch first row
@@ -234,75 +233,6 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
         qout[plane + o] = qn[1];
         qout[2 * plane + o] = qn[2];
       }
-      // xi-boundary ghosts of row j from this row's finished cells
-      if (do_left || do_right) {
-        T s1v[3], s2v[3], s3v[3];
-        if (do_left) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) { s1v[k] = shl1(qn[k]); s2v[k] = shl1(s1v[k]); s3v[k] = shl1(s2v[k]); }
-          if (lane <= 1) {
-            const int kk = lane == 1 ? 1 : 2;  // ghost layer
-            const int gi = 1 - kk;             // ghost column 0 or -1
-            const double xn = centre(M.xi_min, M.dxi, 1), xf = centre(M.xi_min, M.dxi, 2);
-            const double xg = centre(M.xi_min, M.dxi, gi);
-            const long long og = (long long)(j + 1) * pitch + (gi + 1);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              T nr = lane == 1 ? s1v[k] : s2v[k];
-              T fr = lane == 1 ? s2v[k] : s3v[k];
-              qout[k * plane + og] = ghost_value<T>(bc_l, nr, fr, kk, k == 1, xn, xf, xg);
-            }
-          }
-        }
-        if (do_right) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) { s1v[k] = shr1(qn[k]); s2v[k] = shr1(s1v[k]); s3v[k] = shr1(s2v[k]); }
-          if (lane == l_nx + 1 || lane == l_nx + 2) {
-            const int kk = lane - l_nx;
-            const int gi = nx + kk;
-            const double xn = centre(M.xi_min, M.dxi, nx), xf = centre(M.xi_min, M.dxi, nx - 1);
-            const double xg = centre(M.xi_min, M.dxi, gi);
-            const long long og = (long long)(j + 1) * pitch + (gi + 1);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              T nr = kk == 1 ? s1v[k] : s2v[k];
-              T fr = kk == 1 ? s2v[k] : s3v[k];
-              qout[k * plane + og] = ghost_value<T>(bc_r, nr, fr, kk, k == 1, xn, xf, xg);
-            }
-          }
-        }
-      }
-      // eta-boundary ghosts
-      if (j == 1) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) fin1[k] = qn[k];
-      }
-      if (j == 2 && y0 == 1 && out_lane) {
-        const double yn = centre(M.eta_min, M.deta, 1), yf = centre(M.eta_min, M.deta, 2);
-#pragma unroll
-        for (int kk = 1; kk <= 2; ++kk) {
-          const int gj = 1 - kk;
-          const double yg = centre(M.eta_min, M.deta, gj);
-          const long long og = (long long)(gj + 1) * pitch + (i + 1);
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-            qout[k * plane + og] = ghost_value<T>(bc_b, fin1[k], qn[k], kk, k == 2, yn, yf, yg);
-        }
-      }
-      if (j == ny && out_lane) {
-        const double yn = centre(M.eta_min, M.deta, ny), yf = centre(M.eta_min, M.deta, ny - 1);
-#pragma unroll
-        for (int kk = 1; kk <= 2; ++kk) {
-          const int gj = ny + kk;
-          const double yg = centre(M.eta_min, M.deta, gj);
-          const long long og = (long long)(gj + 1) * pitch + (i + 1);
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-            qout[k * plane + og] = ghost_value<T>(bc_t, qn[k], finm[k], kk, k == 2, yn, yf, yg);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) finm[k] = qn[k];
     }
     // shift the window
     Wb = Wa;

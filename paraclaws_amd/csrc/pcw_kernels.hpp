@@ -55,6 +55,11 @@ hipError_t launch_fused_f64(int lim, bool met, const FusedArgs<double>& a, int b
                             int threads, hipStream_t s);
 hipError_t launch_staged_sweep(const StagedArgs& a, const KInst& hI, int dir, int ho,
                                hipStream_t s);
+// buf_sel of launch_ghosts: the buffer whose ghost cells are filled
+constexpr int kGhostsCur = -1;         // the current buffer of every instance
+constexpr int kGhostsAfterFused = 2;   // after an FP64 fused attempt: under host control the
+                                       // pending buffer of a running instance, under device
+                                       // control the current buffer if the attempt was accepted
 template <typename T>
 hipError_t launch_ghosts(T* pool, const KInst* dinst, InstState* dst, int n, int max_side_cells,
                          int buf_sel, hipStream_t s);

@@ -215,8 +215,9 @@ __global__ void k_st_increment(StagedArgs a, int dir) {
 }
 
 // ------------------------------------------------------- ghost fill
-// apply_physical_boundaries (SMOD_boundary_conditions.f90:58-213) on buffer
-// `buf` of every instance; one thread per ghost cell of the four sides.
+// apply_physical_boundaries (SMOD_boundary_conditions.f90:58-213) on the
+// buffer buf_sel selects (kGhosts*) of every instance; one thread per ghost
+// cell of the four sides.  Ghost cells are computed from interior cells only.
 template <typename T>
 __global__ void k_ghosts(T* pool, const KInst* inst, InstState* st, int buf_sel) {
   const KInst& I = inst[blockIdx.y];
@@ -225,7 +226,15 @@ __global__ void k_ghosts(T* pool, const KInst* inst, InstState* st, int buf_sel)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int nlr = 2 * ny, nbt = 2 * nx;
   if (t >= 2 * nlr + 2 * nbt) return;
-  const int b = buf_sel < 0 ? S->cur : (S->cur ^ 1);
+  int b = S->cur;
+  if (buf_sel == kGhostsAfterFused) {
+    if (S->controlled) {
+      if (!S->ghosts_due) return;      // rejected, or no attempt: the current state stays as it is
+    } else {
+      if (S->status != PCW_STATUS_RUNNING) return;
+      b ^= 1;                          // the pending state of pcw_step
+    }
+  }
   T* q = pool + I.off[b];
   const long long pl = I.plane;
   int side, pos, kk;

@@ -22,10 +22,12 @@ __device__ __forceinline__ void finish_attempt(InstState* S, const KInst& I, int
   S->last_smax_y = sy;
   S->last_bad = badbits;
   if (!S->controlled) return;  // host control: the host reads last_smax_* and last_bad
+  S->ghosts_due = 0;
   pcw_step_record rec;
   int ok;
   if (finish_checks(S, sx, sy, I.dxi, I.deta, badbits, &rec, &ok)) return;
   S->cfl_last = rec.cfl;
+  S->ghosts_due = ok;
   if (ok) {
     S->cur = cur ^ 1;
     S->accepted += 1;
