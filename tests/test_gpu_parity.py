@@ -392,3 +392,59 @@ def test_user_bc_rejected():
     with pytest.raises(abi.PcwError) as ei:
         abi.Engine([abi.Domain(16, 16, bc=(1, 2, 2, 2))])
     assert ei.value.code == abi.PCW_ERR_BC
+
+
+RESTORE_MODES = [("fp64", "fused"), ("fp64", "staged"), ("fp32", "fused")]
+
+
+@pytest.mark.parametrize("prec,engine", RESTORE_MODES, ids=[f"{p}-{e}" for p, e in RESTORE_MODES])
+def test_rejected_attempt_leaves_state_and_ghosts(prec, engine):
+    """A rejected attempt leaves the current state as it was, ghost cells
+    included, also when those ghosts are not what the boundary conditions
+    would give (set without a ghost fill): under device control (one
+    attempt, dt0 far above the CFL limit) and under host control (pcw_step,
+    then pcw_reject).  The reference restores solution_old, ghosts included
+    (src/main.f90:213-230)."""
+    abi = _abi()
+    n = 40
+    q = rough_state(n, n, seed=5)  # random ghost cells
+    p = abi.FP32 if prec == "fp32" else abi.FP64
+    eng = abi.STAGED if engine == "staged" else abi.FUSED
+    _say(f"rejected attempt keeps state and ghosts: {prec} {engine}")
+    with abi.Engine([abi.Domain(n, n, bc=(4, 3, 2, 4))], precision=p, engine=eng, limiter=4) as e:
+        e.set_state(0, q)
+        ref = e.get_state(0)  # as stored (FP32: rounded once)
+        e.time_init(t_final=1.0, dt0=0.5, cfl_target=0.9, dt_max=1.0, dt_min=1e-10)
+        e.run(steps_per_graph=1, max_attempts=1)
+        st = e.status(0)
+        assert (st.accepted, st.rejected) == (0, 1)
+        assert np.array_equal(e.get_state(0), ref)
+        try:
+            e.step(0.5)
+        except abi.PcwError as err:  # the eta sweep may meet negative depths at this dt
+            assert err.code == abi.PCW_ERR_STATE_ETA
+        e.reject()
+        assert np.array_equal(e.get_state(0), ref)
+
+
+def test_finished_instance_state_untouched_by_later_runs():
+    """An instance that has finished takes no attempt in later launches, and
+    the FP64 ghost fill that follows each fused attempt leaves its state
+    alone: a state set after it finished (random ghost cells, no ghost fill)
+    is still there, bit for bit, after another run."""
+    abi = _abi()
+    n = 36
+    _say("finished instance untouched: fp64 fused")
+    with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=abi.FP64, limiter=4) as e:
+        e.set_state(0, smooth_state(n, n, seed=3))
+        e.fill_ghosts()
+        e.time_init(t_final=1.0, dt0=1e-4, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10)
+        e.run(steps_per_graph=2, max_accepted=2)
+        st = e.status(0)
+        assert st.status == abi.STATUS_DONE and st.accepted == 2
+        q = rough_state(n, n, seed=9)
+        e.set_state(0, q)
+        e.run(steps_per_graph=2, max_accepted=2)
+        st2 = e.status(0)
+        assert (st2.accepted, st2.rejected) == (st.accepted, st.rejected)
+        assert np.array_equal(e.get_state(0), q)
