@@ -24,6 +24,18 @@
 
 namespace pcw {
 
+// Neighbour lanes in xi through the LDS crossbar (ds_bpermute, two dwords
+// per double) rather than DPP, whose wave-shift forms issue at half the rate
+// of arithmetic on the VALU.  addr = 4 * source lane.  Lane 0 receives lane
+// 63's value and lane 63 lane 0's: both are halo lanes, whose results are
+// never stored and never enter a speed maximum or an error flag.
+__device__ __forceinline__ double bperm64(int addr, double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)u);
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
 template <int LIM, bool MET>
 __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   using T = double;
@@ -55,6 +67,10 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   const T* __restrict__ mq = MET ? a.mpool + I.moff : nullptr;  // kappa, gamma_xi, gamma_eta planes
   const int x0 = t.y, y0 = t.z, y1 = t.w;
   const int i = x0 - 2 + lane;                 // this lane's column (1-based interior)
+  const int adl = ((lane + 63) & 63) << 2;     // left neighbour lane (column i-1)
+  const int adr = ((lane + 1) & 63) << 2;      // right neighbour lane (column i+1)
+  auto shr1 = [adl](double v) { return bperm64(adl, v); };  // value of column i-1
+  auto shl1 = [adr](double v) { return bperm64(adr, v); };  // value of column i+1
   const int fc = i + 1 < nx + 3 ? i + 1 : nx + 3;  // frame column, clamped for loads
   const bool out_lane = lane >= 2 && lane <= 61 && i >= 1 && i <= nx;
   const bool xface = lane >= 1 && i >= 0 && i <= nx + 1;  // interface i-1/2 in the xi CFL range
