@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/pcw.h"
+#include "pcw_device.hpp"
 #include "pcw_engine.hpp"
 #include "pcw_kernels.hpp"
 
@@ -180,12 +181,32 @@ void drop_graph(pcw_handle* h) {
 
 int limiter_code(const pcw_params& p) { return (p.higher_order && p.use_limiter) ? p.limiter : 0; }
 
-// enqueue one fused attempt for every instance (instances with metric arrays
-// and those on the identity map run in separate launches of the kernel)
+// enqueue one fused attempt for every instance.  Instances with metric arrays
+// and those on the identity map run in separate launches of the kernel (FP64:
+// identity-map instances whose metrics are exactly 1 in a third one).
 int enqueue_fused(pcw_handle* h) {
   const int threads = 64 * h->wpb;
   int blocks = (h->ntiles + h->wpb - 1) / h->wpb;
   blocks = (int)round_up(blocks, 8);
+  if (h->p.precision == PCW_FP64) {
+    bool has[3] = {false, false, false};
+    for (const auto& I : h->hinst) has[metric_mode(I)] = true;
+    FusedArgs<double> a{(double*)h->pool, (const double*)h->bpool, (const double*)h->mpool,
+                        h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
+                        h->p.dry_tol > 0.0, 0, h->p.grav, h->p.eps, h->p.dry_tol, nullptr};
+    for (int mode = 0; mode < 3; ++mode) {
+      if (!has[mode]) continue;
+      hipError_t e = launch_fused_f64(limiter_code(h->p), mode, a, blocks, threads, h->stream);
+      if (e != hipSuccess)
+        return fail(h, PCW_ERR_HIP, std::string("fused launch: ") + hipGetErrorString(e));
+    }
+    // ghost cells of the new state
+    hipError_t e = launch_ghosts<double>((double*)h->pool, h->dinst, h->dst, h->n, h->max_side,
+                                         kGhostsAfterFused, h->stream);
+    if (e != hipSuccess)
+      return fail(h, PCW_ERR_HIP, std::string("ghost launch: ") + hipGetErrorString(e));
+    return PCW_OK;
+  }
   // FP32: instances with and without source terms run in separate launches
   const bool wetdry = h->p.dry_tol > 0.0;
   bool any_src = wetdry, any_nosrc = !wetdry;
@@ -193,31 +214,18 @@ int enqueue_fused(pcw_handle* h) {
     any_src = any_src || d.source != PCW_SRC_NONE;
     any_nosrc = any_nosrc || (d.source == PCW_SRC_NONE && !wetdry);
   }
+  FusedArgs<float> a{(float*)h->pool, (const float*)h->bpool, (const float*)h->mpool,
+                     h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
+                     wetdry, 0, h->p.grav, h->p.eps, h->p.dry_tol, h->dslots};
   for (int met = 0; met <= 1; ++met) {
     if (met ? !h->any_met : !h->any_plain) continue;
     hipError_t e = hipSuccess;
-    if (h->p.precision == PCW_FP32) {
-      FusedArgs<float> a{(float*)h->pool, (const float*)h->bpool, (const float*)h->mpool,
-                         h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
-                         wetdry, 0, h->p.grav, h->p.eps, h->p.dry_tol, h->dslots};
-      for (int phys = 0; phys <= 1 && e == hipSuccess; ++phys) {
-        if (phys ? !any_src : !any_nosrc) continue;
-        e = launch_fused_f32(limiter_code(h->p), met, phys, a, blocks, threads, h->stream);
-      }
-    } else {
-      FusedArgs<double> a{(double*)h->pool, (const double*)h->bpool, (const double*)h->mpool,
-                          h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
-                          h->p.dry_tol > 0.0, 0, h->p.grav, h->p.eps, h->p.dry_tol, nullptr};
-      e = launch_fused_f64(limiter_code(h->p), met, a, blocks, threads, h->stream);
+    for (int phys = 0; phys <= 1 && e == hipSuccess; ++phys) {
+      if (phys ? !any_src : !any_nosrc) continue;
+      e = launch_fused_f32(limiter_code(h->p), met, phys, a, blocks, threads, h->stream);
     }
     if (e != hipSuccess)
       return fail(h, PCW_ERR_HIP, std::string("fused launch: ") + hipGetErrorString(e));
-  }
-  if (h->p.precision == PCW_FP64) {  // ghost cells of the new FP64 state
-    hipError_t e = launch_ghosts<double>((double*)h->pool, h->dinst, h->dst, h->n, h->max_side,
-                                         kGhostsAfterFused, h->stream);
-    if (e != hipSuccess)
-      return fail(h, PCW_ERR_HIP, std::string("ghost launch: ") + hipGetErrorString(e));
   }
   return PCW_OK;
 }
@@ -310,6 +318,33 @@ int step_bad_code(pcw_handle* h) {
   return PCW_OK;
 }
 
+// True when the identity map's metrics, rounded as the FP64 kernel computes
+// them (kappa_exact, gamma_xi_exact, gamma_eta_exact), are exactly 1 on every
+// frame cell: the FP64 kernel then skips them (kMetUnit), with the same bits.
+// kappa depends on the column only through the rounded cell width x2 - x1
+// (its other two x-terms are x2 - x2 and x1 - x1, i.e. zero), so it is
+// evaluated on one column per distinct width.
+bool metrics_are_unit(const KInst& I) {
+  const Mesh M{I.xi_min, I.eta_min, I.dxi, I.deta};
+  std::vector<double> widths;
+  std::vector<int> cols;
+  for (int i = -1; i <= I.nx + 2; ++i) {
+    if (gamma_eta_exact(M, i) != 1.0) return false;
+    const double x1 = corner_x1(M, i), w = (x1 + M.dxi) - x1;
+    if (std::find(widths.begin(), widths.end(), w) == widths.end()) {
+      if (widths.size() == 16) return false;
+      widths.push_back(w);
+      cols.push_back(i);
+    }
+  }
+  for (int j = -1; j <= I.ny + 2; ++j) {
+    if (gamma_xi_exact(M, j) != 1.0) return false;
+    for (int i : cols)
+      if (kappa_exact(M, i, j) != 1.0) return false;
+  }
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -381,6 +416,7 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
     I.xi_min = d.xi_min;
     I.eta_min = d.eta_min;
     I.manning = d.manning_n;
+    I.unit_met = p->precision == PCW_FP64 && metrics_are_unit(I);
     {
       auto c = [](double lo, double dd, int i) { return lo + (double)(i - 1) * dd + dd / 2; };
       for (int kk = 1; kk <= 2; ++kk) {

@@ -179,14 +179,14 @@ struct Mesh {
   double xi_min, eta_min, dxi, deta;
 };
 
-__device__ __forceinline__ double corner_x1(const Mesh& m, int i) {
+__host__ __device__ __forceinline__ double corner_x1(const Mesh& m, int i) {
   return m.xi_min + (double)(i - 1) * m.dxi;
 }
-__device__ __forceinline__ double corner_y1(const Mesh& m, int j) {
+__host__ __device__ __forceinline__ double corner_y1(const Mesh& m, int j) {
   return m.eta_min + (double)(j - 1) * m.deta;
 }
 // capacity kappa(i,j)
-__device__ __forceinline__ double kappa_exact(const Mesh& m, int i, int j) {
+__host__ __device__ __forceinline__ double kappa_exact(const Mesh& m, int i, int j) {
   double x1 = corner_x1(m, i), x2 = x1 + m.dxi, x3 = x2, x4 = x1, x5 = x1;
   double y1 = corner_y1(m, j), y2 = y1, y3 = y2 + m.deta, y4 = y3, y5 = y1;
   double a = 0.0;
@@ -197,19 +197,19 @@ __device__ __forceinline__ double kappa_exact(const Mesh& m, int i, int j) {
   return a / (m.dxi * m.deta);
 }
 // length_ratio_xi(1,i,j): depends on j only (face 1->4 is vertical)
-__device__ __forceinline__ double gamma_xi_exact(const Mesh& m, int j) {
+__host__ __device__ __forceinline__ double gamma_xi_exact(const Mesh& m, int j) {
   double y1 = corner_y1(m, j), y3 = y1 + m.deta;
   double fdx = 0.0, fdy = y3 - y1;
   return sqrt(fdx * fdx + fdy * fdy) / m.deta;
 }
 // length_ratio_eta(1,i,j): depends on i only (face 1->2 is horizontal)
-__device__ __forceinline__ double gamma_eta_exact(const Mesh& m, int i) {
+__host__ __device__ __forceinline__ double gamma_eta_exact(const Mesh& m, int i) {
   double x1 = corner_x1(m, i), x2 = x1 + m.dxi;
   double fdx = x2 - x1, fdy = 0.0;
   return sqrt(fdx * fdx + fdy * fdy) / m.dxi;
 }
 // cell centre, SMOD_mesh_manipulation.f90:50-51
-__device__ __forceinline__ double centre(double lo, double d, int i) {
+__host__ __device__ __forceinline__ double centre(double lo, double d, int i) {
   return lo + (double)(i - 1) * d + d / 2;
 }
 

@@ -37,6 +37,7 @@ struct KInst {
   int source, tile0;   // tile0: index of the instance's first tile in the tile list
   int slot_rows;       // 1: frame rows in 64-element slots (FP32 engine), 0: plain rows
   int nslot;           // slot rows: interior slots, ceil(nx / kStripOut)
+  int unit_met;        // identity map whose rounded metrics are all exactly 1 (FP64 engine)
   long long plane;     // elements per field plane (pitch * (ny+4))
   long long off[2];    // element offset of the h-plane frame origin (i=-1,j=-1) of buffer 0/1
   long long boff;      // bathymetry frame offset, -1 when absent

@@ -36,7 +36,7 @@ __device__ __forceinline__ double bperm64(int addr, double v) {
   return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-template <int LIM, bool MET>
+template <int LIM, int MET>
 __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   using T = double;
   const int lane = threadIdx.x & 63;
@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   const int in = __builtin_amdgcn_readfirstlane(t.x);
   const KInst& I = a.inst[in];
   InstState* S = a.st + in;
-  if (MET != (I.moff >= 0)) return;  // instances with and without metric arrays run in separate launches
+  if (MET != metric_mode(I)) return;  // each kind of metric handling has a launch of its own
   if (S->status != PCW_STATUS_RUNNING) {
     // no attempt this launch: the ghost fill that follows must not run
     if (tid == I.tile0 && lane == 0) S->ghosts_due = 0;
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   const double dt = S->tc.dt;
   const T* __restrict__ qin = a.pool + I.off[cur];
   T* __restrict__ qout = a.pool + I.off[cur ^ 1];
-  const T* __restrict__ mq = MET ? a.mpool + I.moff : nullptr;  // kappa, gamma_xi, gamma_eta planes
+  const T* __restrict__ mq = MET == kMetArrays ? a.mpool + I.moff : nullptr;  // kappa, gamma_xi, gamma_eta planes
   const int x0 = t.y, y0 = t.z, y1 = t.w;
   const int i = x0 - 2 + lane;                 // this lane's column (1-based interior)
   const int adl = ((lane + 63) & 63) << 2;     // left neighbour lane (column i-1)
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   const T* __restrict__ bb_ = (I.boff >= 0) ? a.bpool + I.boff : nullptr;
   const bool use_b = bb_ && (src == PCW_SRC_BATHY || src == PCW_SRC_BATHY_FRICTION);
   using MT = double;
-  const MT gam_eta_col = gamma_eta_exact(M, i);
+  const MT gam_eta_col = MET == kMetUnit ? MT(1) : gamma_eta_exact(M, i);
 
   T smx = T(0), smy = T(0);
   int bad = 0;   // bit 1: NaN/complex in the xi sweep, bit 2: in the eta sweep
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   long long roff = (long long)(y0 - 2 + 1) * pitch + fc;
   T nh = qin[roff], nhu = qin[plane + roff], nhv = qin[2 * plane + roff];
   T nk = T(1), ngx = T(1), ngy = T(1);
-  if (MET) {
+  if (MET == kMetArrays) {
     nk = mq[roff];
     ngx = mq[plane + roff];
     ngy = mq[2 * plane + roff];
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
       nh = qin[o];
       nhu = qin[plane + o];
       nhv = qin[2 * plane + o];
-      if (MET) {
+      if (MET == kMetArrays) {
         nk = mq[o];
         ngx = mq[plane + o];
         ngy = mq[2 * plane + o];
@@ -139,8 +139,8 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
     T bnew = T(0);
     if (use_b && r >= y0) bnew = bb_[(long long)r * pitch + fc];  // b(i, r-1)
 
-    const MT kap_r = MET ? mk_r : kappa_exact(M, i, r);
-    const MT gy_r = MET ? (MT)mgy_r : gam_eta_col;
+    const MT kap_r = MET == kMetArrays ? mk_r : (MET == kMetUnit ? MT(1) : kappa_exact(M, i, r));
+    const MT gy_r = MET == kMetArrays ? (MT)mgy_r : gam_eta_col;
     T qx[3] = {q[0], q[1], q[2]};
 
     // ------------------------------------------------------ xi sweep
@@ -161,9 +161,9 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) w1R[k] = shl1(w.w1[k]);
       // metric neighbours (all lane shifts in wave-uniform control flow)
-      const MT kap_l = shr1(kap_r);                             // kappa(i-1, r)
-      const MT gx_l = MET ? mgx_r : gamma_xi_exact(M, r);
-      const MT gx_r = MET ? shl1(gx_l) : gx_l;                  // gamma_xi(i+1, r)
+      const MT kap_l = MET == kMetUnit ? MT(1) : shr1(kap_r);  // kappa(i-1, r)
+      const MT gx_l = MET == kMetArrays ? mgx_r : (MET == kMetUnit ? MT(1) : gamma_xi_exact(M, r));
+      const MT gx_r = MET == kMetArrays ? shl1(gx_l) : gx_l;    // gamma_xi(i+1, r)
       if (a.ho) {
         T w2L[3];
 #pragma unroll
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
 }
 
 // ------------------------------------------------------------- launcher
-template <bool MET>
+template <int MET>
 static hipError_t launch_lim(int lim, const FusedArgs<double>& a, int blocks, int threads,
                              hipStream_t s) {
   switch (lim) {
@@ -288,10 +288,13 @@ static hipError_t launch_lim(int lim, const FusedArgs<double>& a, int blocks, in
   return hipGetLastError();
 }
 
-hipError_t launch_fused_f64(int lim, bool met, const FusedArgs<double>& a, int blocks, int threads,
-                            hipStream_t s) {
-  return met ? launch_lim<true>(lim, a, blocks, threads, s)
-             : launch_lim<false>(lim, a, blocks, threads, s);
+hipError_t launch_fused_f64(int lim, int met_mode, const FusedArgs<double>& a, int blocks,
+                            int threads, hipStream_t s) {
+  switch (met_mode) {
+    case kMetArrays: return launch_lim<kMetArrays>(lim, a, blocks, threads, s);
+    case kMetUnit: return launch_lim<kMetUnit>(lim, a, blocks, threads, s);
+    default: return launch_lim<kMetComputed>(lim, a, blocks, threads, s);
+  }
 }
 
 }  // namespace pcw

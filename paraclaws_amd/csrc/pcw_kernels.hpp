@@ -51,7 +51,16 @@ hipError_t launch_fused_f32(int lim, bool met, bool phys, const FusedArgs<float>
                             int threads, hipStream_t s);
 // resident blocks per CU of the FP32 fused kernel variant (0 if unknown)
 int fused_f32_blocks_per_cu(int lim, bool ho, bool met, bool phys, int threads);
-hipError_t launch_fused_f64(int lim, bool met, const FusedArgs<double>& a, int blocks,
+// Mesh metrics in an FP64 fused launch: identity-map instances with the
+// reference's rounded metrics computed in the kernel, instances with metric
+// arrays, and identity-map instances whose rounded metrics are all exactly 1
+// (multiplying and dividing by them changes no bit, so the kernel skips them).
+// Each launch runs the instances of one kind.
+constexpr int kMetComputed = 0, kMetArrays = 1, kMetUnit = 2;
+__host__ __device__ inline int metric_mode(const KInst& I) {
+  return I.moff >= 0 ? kMetArrays : (I.unit_met ? kMetUnit : kMetComputed);
+}
+hipError_t launch_fused_f64(int lim, int met_mode, const FusedArgs<double>& a, int blocks,
                             int threads, hipStream_t s);
 hipError_t launch_staged_sweep(const StagedArgs& a, const KInst& hI, int dir, int ho,
                                hipStream_t s);
