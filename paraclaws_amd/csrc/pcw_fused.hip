@@ -37,7 +37,7 @@ __device__ __forceinline__ double bperm64(int addr, double v) {
 }
 
 template <int LIM, int MET>
-__global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
+__device__ __forceinline__ void fused_step(const FusedArgs<double>& a) {
   using T = double;
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
@@ -274,16 +274,38 @@ __global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
   tile_finish(S, I, cur, (double)mx, (double)my, bad_x != 0, bad_y != 0, lane);
 }
 
+template <int LIM, int MET>
+__global__ void __launch_bounds__(256) k_fused_step(FusedArgs<double> a) {
+  fused_step<LIM, MET>(a);
+}
+
+// Without the metric arithmetic the step fits 128 VGPRs: four waves per SIMD.
+template <int LIM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_fused_step_unit(FusedArgs<double> a) {
+  fused_step<LIM, kMetUnit>(a);
+}
+
 // ------------------------------------------------------------- launcher
 template <int MET>
 static hipError_t launch_lim(int lim, const FusedArgs<double>& a, int blocks, int threads,
                              hipStream_t s) {
-  switch (lim) {
-    case 0: k_fused_step<0, MET><<<blocks, threads, 0, s>>>(a); break;
-    case 1: k_fused_step<1, MET><<<blocks, threads, 0, s>>>(a); break;
-    case 2: k_fused_step<2, MET><<<blocks, threads, 0, s>>>(a); break;
-    case 3: k_fused_step<3, MET><<<blocks, threads, 0, s>>>(a); break;
-    default: k_fused_step<4, MET><<<blocks, threads, 0, s>>>(a); break;
+  if constexpr (MET == kMetUnit) {
+    switch (lim) {
+      case 0: k_fused_step_unit<0><<<blocks, threads, 0, s>>>(a); break;
+      case 1: k_fused_step_unit<1><<<blocks, threads, 0, s>>>(a); break;
+      case 2: k_fused_step_unit<2><<<blocks, threads, 0, s>>>(a); break;
+      case 3: k_fused_step_unit<3><<<blocks, threads, 0, s>>>(a); break;
+      default: k_fused_step_unit<4><<<blocks, threads, 0, s>>>(a); break;
+    }
+  } else {
+    switch (lim) {
+      case 0: k_fused_step<0, MET><<<blocks, threads, 0, s>>>(a); break;
+      case 1: k_fused_step<1, MET><<<blocks, threads, 0, s>>>(a); break;
+      case 2: k_fused_step<2, MET><<<blocks, threads, 0, s>>>(a); break;
+      case 3: k_fused_step<3, MET><<<blocks, threads, 0, s>>>(a); break;
+      default: k_fused_step<4, MET><<<blocks, threads, 0, s>>>(a); break;
+    }
   }
   return hipGetLastError();
 }
