@@ -68,9 +68,47 @@ struct CellQ {
   int bad;  // 1: NaN in J or complex eigenvalues (reference error stop)
 };
 
+// ------------------------------------------------ quotients by one divisor
+// gfx950 divides in FP64 with v_div_scale (scaling when an operand lies near
+// the ends of the exponent range), v_rcp_f64 and two Newton steps on the
+// reciprocal of the divisor, the quotient with one fma correction
+// (v_div_fmas), then v_div_fixup (zero, infinite and NaN operands).  When no
+// scaling applies, the reciprocal depends on the divisor alone, so quotients
+// with one divisor can share it: each then costs a multiply, two fmas and the
+// fixup, and has the bits of the full division.  No scaling applies when the
+// divisor's binary exponent (frexp) lies in [-299, 300] and the dividend is
+// zero or has its exponent in [-700, 440]: the exponent difference stays
+// below 768, the divisor, its reciprocal and the quotient are normal, and
+// the dividend is far from the denormal range.  For zero, infinite and NaN
+// operands v_div_fixup returns its answer whatever the estimate.
+struct Divisor {
+  double d, r;
+};
+__device__ __forceinline__ Divisor divisor(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  return {d, r};
+}
+__device__ __forceinline__ double quot(double n, const Divisor& D) {
+  const double q = n * D.r;
+  const double rem = fma(-D.d, q, n);
+  return __builtin_amdgcn_div_fixup(fma(rem, D.r, q), D.d, n);
+}
+// frexp exponent of x (0 for zero) within [lo, hi]
+__device__ __forceinline__ bool exp_in(double x, int lo, int hi) {
+  return (unsigned)(__builtin_amdgcn_frexp_exp(x) - lo) <= (unsigned)(hi - lo);
+}
+
 // --------------------------------------------------------- Exact (double)
 // Cell physics for the xi sweep (dir 1) or eta sweep (dir 2):
 // SMOD_jakobian.f90:44-53 entries, eigenvalues {u-c,u,u+c}, J*q column order.
+// When every lane of the wave has h in [2^-150, 2^150) and hu, hv zero or in
+// [2^-350, 2^220), the divisors h and h*h and the dividends hu, hv, hu*hu,
+// hu*hv, hv*hv are in the range where quotients share a reciprocal; then
+// also 2*hu/h = 2*(hu/h) exactly.
 template <int DIR>
 __device__ __forceinline__ CellQ<double> cell_exact(double h, double hu, double hv,
                                                     const Phys& P) {
@@ -90,23 +128,44 @@ __device__ __forceinline__ CellQ<double> cell_exact(double h, double hu, double 
   } else {
     cc = sqrt(gh);
   }
+  const bool shared = __all(exp_in(h, -149, 150) && exp_in(hu, -349, 220) && exp_in(hv, -349, 220));
   if (DIR == 1) {
-    double J21 = -(hu * hu / (h * h)) + g * h;
-    double J22 = 2.0 * hu / h;
-    double J31 = -(hu * hv / (h * h));
-    double J32 = hv / h;
-    double J33 = hu / h;
+    double J21, J22, J31, J32, J33;
+    if (shared) {
+      const Divisor Dh = divisor(h), Dhh = divisor(h * h);
+      J21 = -quot(hu * hu, Dhh) + g * h;
+      J33 = quot(hu, Dh);
+      J22 = 2.0 * J33;
+      J31 = -quot(hu * hv, Dhh);
+      J32 = quot(hv, Dh);
+    } else {
+      J21 = -(hu * hu / (h * h)) + g * h;
+      J22 = 2.0 * hu / h;
+      J31 = -(hu * hv / (h * h));
+      J32 = hv / h;
+      J33 = hu / h;
+    }
     if (isnan(J21) || isnan(J22) || isnan(J31) || isnan(J32) || isnan(J33)) r.bad = 1;
     r.s = fabs(J33) + cc;
     r.f0 = hu;
     r.f1 = J21 * h + J22 * hu;
     r.f2 = (J31 * h + J32 * hu) + J33 * hv;
   } else {
-    double J21 = -(hu * hv / (h * h));
-    double J22 = hv / h;
-    double J23 = hu / h;
-    double J31 = -(hv * hv / (h * h)) + g * h;
-    double J33 = 2.0 * hv / h;
+    double J21, J22, J23, J31, J33;
+    if (shared) {
+      const Divisor Dh = divisor(h), Dhh = divisor(h * h);
+      J21 = -quot(hu * hv, Dhh);
+      J22 = quot(hv, Dh);
+      J23 = quot(hu, Dh);
+      J31 = -quot(hv * hv, Dhh) + g * h;
+      J33 = 2.0 * J22;
+    } else {
+      J21 = -(hu * hv / (h * h));
+      J22 = hv / h;
+      J23 = hu / h;
+      J31 = -(hv * hv / (h * h)) + g * h;
+      J33 = 2.0 * hv / h;
+    }
     if (isnan(J21) || isnan(J22) || isnan(J23) || isnan(J31) || isnan(J33)) r.bad = 1;
     r.s = fabs(J22) + cc;
     r.f0 = hv;
