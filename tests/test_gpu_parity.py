@@ -77,6 +77,52 @@ def test_fp64_step_bitwise(oracle, engine, case):
     assert cfl == cfl_ref
 
 
+def _operand_range_state(nx, ny, seed):
+    """A smooth state whose momenta cover every magnitude class the FP64 cell
+    physics tells apart, in row bands: plain values; exact zeros of both
+    signs; values scaled to ~1e-100 (still where quotients by h and h*h share
+    a reciprocal); and sprinkled tiny normals and denormals (1e-110 .. 5e-324,
+    outside that range), so that some waves take each path."""
+    rng = np.random.default_rng(seed)
+    q = smooth_state(nx, ny, seed=seed)
+    b = (ny + 4) // 4
+    zeros = np.where(rng.random((b, nx + 4)) < 0.5, 0.0, -0.0)
+    q[b:2 * b, :, 1] = zeros
+    q[b:2 * b, :, 2] = np.where(rng.random((b, nx + 4)) < 0.5, -0.0, q[b:2 * b, :, 2])
+    q[2 * b:3 * b, :, 1:] *= 1e-100
+    tiny = np.array([1e-110, -1e-200, 1e-300, 5e-324, -3e-320, 2.0 ** -349, -(2.0 ** -351)])
+    sel = rng.random((ny + 4 - 3 * b, nx + 4, 2)) < 0.3
+    q[3 * b:, :, 1:] = np.where(sel, rng.choice(tiny, size=sel.shape), q[3 * b:, :, 1:])
+    return q
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_fp64_step_bitwise_operand_ranges(oracle, engine):
+    """One step from a state with momenta of every magnitude class (see
+    _operand_range_state) equals the oracle bit for bit, signs of zero
+    included."""
+    abi = _abi()
+    O = oracle
+    nx, ny = 130, 52
+    q0 = _operand_range_state(nx, ny, seed=21)
+    cfg = O.Config(nx=nx, ny=ny, limiter=4, bc=(4, 3, 2, 4))
+    O.fill_ghosts(cfg, q0)
+    dt = stable_dt(q0, nx, ny)
+    ref = q0.copy()
+    cfl_ref = _oracle_step(O, cfg, ref, dt)
+    _say(f"fp64 {engine} step, operand ranges")
+    with abi.Engine([abi.Domain(nx, ny, bc=cfg.bc)], precision=abi.FP64,
+                    engine=abi.FUSED if engine == "fused" else abi.STAGED, limiter=4) as e:
+        e.set_state(0, q0)
+        smax = e.step(dt)[0]
+        e.accept()
+        got = e.get_state(0)
+    m = sides_mask(nx, ny)
+    assert np.array_equal(got[m].view(np.uint64), ref[m].view(np.uint64))
+    cfl = max(smax[0] * dt / (2.0 / nx), smax[1] * dt / (2.0 / ny))
+    assert cfl == cfl_ref
+
+
 FP32_CASES = CASES + [
     # grids with interior strips and long tiles, so that the FP32 kernel's
     # branch-free runs of rows are used; the last field is tile_rows
