@@ -192,11 +192,23 @@ __device__ __forceinline__ Waves<double> hll_exact(const double qL[3], const Cel
   if (fabs(s1) > eps && fabs(s2) > eps) {
     double den = eps + s1 - s2;
     double fL[3] = {cL.f0, cL.f1, cL.f2}, fR[3] = {cR.f0, cR.f1, cR.f2};
+    double num[3], qs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) num[k] = fR[k] - fL[k] - s2 * qR[k] + s1 * qL[k];
+    // the three quotients share a reciprocal when the wave's operands allow
+    if (__all(exp_in(den, -299, 300) && exp_in(num[0], -700, 440) && exp_in(num[1], -700, 440) &&
+              exp_in(num[2], -700, 440))) {
+      const Divisor D = divisor(den);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) qs[k] = quot(num[k], D);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) qs[k] = num[k] / den;
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      double qs = (fR[k] - fL[k] - s2 * qR[k] + s1 * qL[k]) / den;
-      w.w1[k] = qs - qL[k];
-      w.w2[k] = qR[k] - qs;
+      w.w1[k] = qs[k] - qL[k];
+      w.w2[k] = qR[k] - qs[k];
     }
     w.s2 = s2;
   } else {
