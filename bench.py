@@ -294,6 +294,8 @@ def main():
                     help="skip the bathymetry (8192^2) and friction-batch legs")
     ap.add_argument("--profile-only", action="store_true",
                     help="FP32 fused run only (for rocprofv3 kernel traces and counters)")
+    ap.add_argument("--profile-fp64", action="store_true",
+                    help="FP64 fused (bit-exact) run only, for rocprofv3 kernel traces")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB (FETCH/WRITE counter scale)")
     args = ap.parse_args()
@@ -318,6 +320,16 @@ def main():
 
     n = args.n
     cells = n * n
+    if args.profile_fp64:
+        log(f"FP64 fused engine, dam break {n}^2, {args.steps} accepted steps")
+        ef, rf = run_engine(abi, ic, n, abi.FP64, abi.FUSED, args.steps, args.warmup,
+                            args.steps_per_graph, not args.no_autotune, repeat=args.repeat,
+                            tiles=args.tiles)
+        ef.close()
+        print(json.dumps({"value": rf["accepted"] * cells / rf["wall_s"],
+                          "step_avg_ms": rf["avg_step_ms"], "tuned": rf["tuned"],
+                          "steps": rf["accepted"]}), flush=True)
+        return
     log(f"FP32 fused engine, dam break {n}^2, {args.steps} accepted steps, "
         f"median of {args.repeat} runs")
     e32, r32 = run_engine(abi, ic, n, abi.FP32, abi.FUSED, args.steps, args.warmup,
