@@ -1,13 +1,14 @@
 // pcw_fused.hip -- FP64 step kernel: xi sweep, eta sweep and source term of
 // one time step, for any number of instances at once.  The physical ghost
 // fill of the new state follows in a small kernel of its own (k_ghosts), so
-// that this one stays within 168 VGPRs (three waves per SIMD).
+// that this one stays within 168 VGPRs (three waves per SIMD; four, within
+// 128 VGPRs, for instances whose mesh metrics are exactly 1).
 //
-// Mapping (gfx950, wave64, no LDS): a wave owns a strip of 60 output
-// columns plus two halo lanes on each side and marches down the rows of its
-// tile.  Each row is loaded once (coalesced SoA h/hu/hv), swept in xi with
-// DPP lane shifts carrying the neighbour states, and fed into a register
-// window that runs the eta sweep two rows behind.  The eta sweep reads the
+// Mapping (gfx950, wave64): a wave owns a strip of 60 output columns plus
+// two halo lanes on each side and marches down the rows of its tile.  Each
+// row is loaded once (coalesced SoA h/hu/hv), swept in xi with lane shifts
+// through the LDS crossbar carrying the neighbour states, and fed into a
+// register window that runs the eta sweep two rows behind.  The eta sweep reads the
 // xi-updated interior and the ghost rows as they were before the step
 // (the reference refreshes ghosts once per step, src/main.f90:246).
 // Results go to the instance's other buffer (ping-pong), so a rejected
@@ -69,8 +70,8 @@ __device__ __forceinline__ void fused_step(const FusedArgs<double>& a) {
   const int i = x0 - 2 + lane;                 // this lane's column (1-based interior)
   const int adl = ((lane + 63) & 63) << 2;     // left neighbour lane (column i-1)
   const int adr = ((lane + 1) & 63) << 2;      // right neighbour lane (column i+1)
-  auto shr1 = [adl](double v) { return bperm64(adl, v); };  // value of column i-1
-  auto shl1 = [adr](double v) { return bperm64(adr, v); };  // value of column i+1
+  auto from_left = [adl](double v) { return bperm64(adl, v); };   // value of column i-1
+  auto from_right = [adr](double v) { return bperm64(adr, v); };  // value of column i+1
   const int fc = i + 1 < nx + 3 ? i + 1 : nx + 3;  // frame column, clamped for loads
   const bool out_lane = lane >= 2 && lane <= 61 && i >= 1 && i <= nx;
   const bool xface = lane >= 1 && i >= 0 && i <= nx + 1;  // interface i-1/2 in the xi CFL range
@@ -148,26 +149,26 @@ __device__ __forceinline__ void fused_step(const FusedArgs<double>& a) {
       CellQ<T> c;
       c = cell_exact<1>(q[0], q[1], q[2], P);
       bad |= (c.bad && check_cell) ? 1 : 0;
-      T qL[3] = {shr1(q[0]), shr1(q[1]), shr1(q[2])};
+      T qL[3] = {from_left(q[0]), from_left(q[1]), from_left(q[2])};
       CellQ<T> cL;
-      cL.s = shr1(c.s);
+      cL.s = from_left(c.s);
       cL.f0 = qL[1];
-      cL.f1 = shr1(c.f1);
-      cL.f2 = shr1(c.f2);
+      cL.f1 = from_left(c.f1);
+      cL.f2 = from_left(c.f2);
       Waves<T> w;
       w = hll_exact(qL, cL, q, c, P.eps);
       smx = (xface && w.s2 > smx) ? w.s2 : smx;
       T w1R[3], F[3] = {T(0), T(0), T(0)};
 #pragma unroll
-      for (int k = 0; k < 3; ++k) w1R[k] = shl1(w.w1[k]);
+      for (int k = 0; k < 3; ++k) w1R[k] = from_right(w.w1[k]);
       // metric neighbours (all lane shifts in wave-uniform control flow)
-      const MT kap_l = MET == kMetUnit ? MT(1) : shr1(kap_r);  // kappa(i-1, r)
+      const MT kap_l = MET == kMetUnit ? MT(1) : from_left(kap_r);  // kappa(i-1, r)
       const MT gx_l = MET == kMetArrays ? mgx_r : (MET == kMetUnit ? MT(1) : gamma_xi_exact(M, r));
-      const MT gx_r = MET == kMetArrays ? shl1(gx_l) : gx_l;    // gamma_xi(i+1, r)
+      const MT gx_r = MET == kMetArrays ? from_right(gx_l) : gx_l;    // gamma_xi(i+1, r)
       if (a.ho) {
         T w2L[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) w2L[k] = shr1(w.w2[k]);
+        for (int k = 0; k < 3; ++k) w2L[k] = from_left(w.w2[k]);
         T p1 = T(1), p2 = T(1);
         if (LIM > 0 && xlim) {
           p1 = limit_exact<LIM>(w.w1, w1R, P.eps);
@@ -182,10 +183,10 @@ __device__ __forceinline__ void fused_step(const FusedArgs<double>& a) {
           F[k] = t1 + t2;
         }
       }
-      const T s2R = shl1(w.s2);
+      const T s2R = from_right(w.s2);
       T FR[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) FR[k] = shl1(F[k]);
+      for (int k = 0; k < 3; ++k) FR[k] = from_right(F[k]);
       if (i >= 1 && i <= nx) {
         const double fac = dtod_x / kap_r;
 #pragma unroll
@@ -239,7 +240,7 @@ __device__ __forceinline__ void fused_step(const FusedArgs<double>& a) {
         qn[k] = a.ho ? q1 - dtod_y / kap_pp * (Fa[k] - Fb[k]) : q1;
       }
       if (do_src) {
-        T bxm = shr1(b0), bxp = shl1(b0);
+        T bxm = from_left(b0), bxp = from_right(b0);
         apply_source<T>(qn[0], qn[1], qn[2], src, dry, g, (T)I.dxi, (T)I.deta, (T)dt, n2, bxm,
                         bxp, bm, bnew);
       }
