@@ -19,6 +19,8 @@ namespace pcw {
 
 // --------------------------------------------------------------- lane shifts
 // DPP wave_shr:1 / wave_shl:1 (GFX9 family): lane l receives lane l-1 / l+1.
+// Used by the FP32 kernel off its per-row path (metric and bathymetry
+// neighbours).
 // Lanes shifted in from outside the wave receive 0 (bound_ctrl), so every
 // lane is written and no old value has to be set up.
 __device__ __forceinline__ int shr1_i(int v) {
@@ -32,16 +34,6 @@ __device__ __forceinline__ float shr1(float v) {
 }
 __device__ __forceinline__ float shl1(float v) {
   return __builtin_bit_cast(float, shl1_i(__builtin_bit_cast(int, v)));
-}
-__device__ __forceinline__ double shr1(double v) {
-  uint64_t u = __builtin_bit_cast(uint64_t, v);
-  int lo = shr1_i((int)(uint32_t)u), hi = shr1_i((int)(uint32_t)(u >> 32));
-  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-}
-__device__ __forceinline__ double shl1(double v) {
-  uint64_t u = __builtin_bit_cast(uint64_t, v);
-  int lo = shl1_i((int)(uint32_t)u), hi = shl1_i((int)(uint32_t)(u >> 32));
-  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
 // wave-wide max of a non-negative value (butterfly over 64 lanes)
