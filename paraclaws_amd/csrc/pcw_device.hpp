@@ -19,8 +19,7 @@ namespace pcw {
 
 // --------------------------------------------------------------- lane shifts
 // DPP wave_shr:1 / wave_shl:1 (GFX9 family): lane l receives lane l-1 / l+1.
-// Used by the FP32 kernel off its per-row path (metric and bathymetry
-// neighbours).
+// The FP32 kernel takes its metric and bathymetry neighbours this way.
 // Lanes shifted in from outside the wave receive 0 (bound_ctrl), so every
 // lane is written and no old value has to be set up.
 __device__ __forceinline__ int shr1_i(int v) {
