@@ -246,9 +246,10 @@ __device__ __forceinline__ void fused_step(const FusedArgs<double>& a) {
       }
       const long long o = (long long)(j + 1) * pitch + (i + 1);
       if (out_lane) {
-        qout[o] = qn[0];
-        qout[plane + o] = qn[1];
-        qout[2 * plane + o] = qn[2];
+        // non-temporal, as the FP32 kernel's stores: nothing here reads them
+        __builtin_nontemporal_store(qn[0], qout + o);
+        __builtin_nontemporal_store(qn[1], qout + plane + o);
+        __builtin_nontemporal_store(qn[2], qout + 2 * plane + o);
       }
     }
     // shift the window

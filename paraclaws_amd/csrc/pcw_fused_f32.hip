@@ -79,8 +79,16 @@ __device__ __forceinline__ rsrc_t make_rsrc(const float* base, int bytes) {
 __device__ __forceinline__ float bload(rsrc_t r, unsigned voff, int soff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
 }
+// The new state is stored non-temporal (cache policy nt): the kernel never
+// reads what it stores, and the next step reads it back only after the XCDs'
+// L2s (4 MB each) have turned over many times (a step moves 1.6 GB at
+// 8192^2).  With nt the 8192^2 step runs 4.5 % faster (391 -> 373 us).  Loads
+// keep the default policy: neighbouring strips and tiles re-read their halos
+// from L2 (nt loads made the step 518 us).
+constexpr int kStoreNT = 2;  // gfx950 buffer-store cache policy bit nt
 __device__ __forceinline__ void bstore(float v, rsrc_t r, unsigned voff, int soff) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, soff,
+                                        kStoreNT);
 }
 
 // One row's cell values and the interface below it, carried by the eta window.
