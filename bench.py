@@ -264,14 +264,15 @@ def cpu_baseline(n, target_s=15.0):
 
 def load_traffic(n):
     """HBM bytes per launch of the fused FP32 step from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE runs, if they exist for this grid size and kernel."""
+    FETCH_SIZE/WRITE_SIZE runs, if they exist for this grid size and kernel,
+    with the tiles (rows, waves per block) they were measured at."""
     p = os.path.join(ROOT, "profiles", f"pmc_fused_step_f32_{n}.json")
     if os.path.exists(p):
         with open(p) as f:
             rec = json.load(f)
         if rec.get("kernel") == KERNEL_F32:
-            return rec.get("traffic_bytes_per_launch")
-    return None
+            return rec.get("traffic_bytes_per_launch"), [rec.get("tile_rows"), rec.get("waves_per_block")]
+    return None, None
 
 
 def main():
@@ -340,7 +341,7 @@ def main():
         f"kernel avg {r32['avg_step_ms']:.4f} ms  rejected {r32['rejected']}")
     kernel_ms = r32["avg_step_ms"]
     achieved = BYTES_PER_CELL_UPDATE_F32 * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
-    traffic = load_traffic(n)
+    traffic, traffic_tiles = load_traffic(n)
     e32.close()
     if args.profile_only:
         print(json.dumps({"value": value, "kernel_avg_ms": kernel_ms, "achieved_gbs": achieved,
@@ -445,6 +446,7 @@ def main():
                    "tile_rows_waves": r32["tuned"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_tile_rows_waves": traffic_tiles,
                      "kernel": KERNEL_F32, "kernel_avg_ms": kernel_ms,
                      "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_F32},
         "cpu_baseline": cpu,
