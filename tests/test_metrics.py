@@ -131,6 +131,38 @@ def test_gpu_mixed_batch_mapped_and_plain(oracle):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nx,ny", [(64, 32), (37, 29)], ids=["unit-metrics", "computed-metrics"])
+def test_gpu_fp64_metric_kinds_agree(oracle, nx, ny):
+    """The FP64 fused kernel on the identity map -- with the metrics skipped
+    (all exactly 1, 64 x 32) or computed in the kernel (37 x 29) -- equals the
+    same grid run with the identity map's metrics passed as arrays, bit for
+    bit, over a device-controlled run."""
+    from paraclaws_amd import abi
+
+    cfg = oracle.Config(nx=nx, ny=ny, limiter=4, bc=(4, 3, 2, 4))
+    m = oracle.metrics(cfg)
+    unit = all(bool(np.all(m[k] == 1.0)) for k in ("kappa", "gamma_xi", "gamma_eta"))
+    assert unit == ((nx, ny) == (64, 32))
+    q0 = rough_state(nx, ny, seed=nx + ny)
+    oracle.fill_ghosts(cfg, q0)
+    tkw = dict(t_final=0.02, dt0=1e-4, n_snapshots=2)
+    out = []
+    for arrays in (False, True):
+        print(f"fp64 metric kinds {nx}x{ny} arrays={arrays}", flush=True)
+        with abi.Engine([abi.Domain(nx, ny, bc=cfg.bc)], precision=abi.FP64, limiter=4) as e:
+            e.set_state(0, q0)
+            if arrays:
+                e.set_metrics(0, m["kappa"], m["gamma_xi"], m["gamma_eta"])
+            e.time_init(**tkw)
+            e.run(steps_per_graph=8)
+            st = e.status(0)
+            out.append((e.get_state(0), st.accepted, st.rejected))
+    assert out[0][1:] == out[1][1:]
+    msk = sides_mask(nx, ny)
+    assert np.array_equal(out[0][0][msk].view(np.uint64), out[1][0][msk].view(np.uint64))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mapping", [STRETCH, CURVED], ids=["stretch", "curved"])
 def test_gpu_fp32_mapped_close(oracle, mapping):
     """FP32 fused step with metric arrays (FP32 frames in 64-element slot rows,
