@@ -614,7 +614,8 @@ k_step(FusedArgs<float> a) {
   float fin1[3] = {0.0f, 0.0f, 0.0f}, finm[3] = {0.0f, 0.0f, 0.0f};
   float bm = 0.0f, b0 = 0.0f;
 
-  // prefetch rows y0-2 .. y0-2+kPF-1 (a tile has at least 7 rows to march)
+  // prefetch rows y0-2 .. y0-2+kPF-1 (clamped to y1+2: a tile of a two-row
+  // domain marches six rows)
   constexpr int kPF = Ring<MET, PHYS>::kPF;
   Pre P[kPF];
 #pragma unroll

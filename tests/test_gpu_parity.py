@@ -47,6 +47,10 @@ CASES = [
     (125, 9, 4, True, (2, 4, 3, 2), "smooth"),
     (8, 200, 4, False, (3, 2, 4, 3), "rough"),
     (70, 70, 0, True, (4, 4, 4, 4), "rough"),
+    # the smallest grids the ABI accepts (nx, ny >= 2) and a one-strip, two-row one
+    (2, 2, 4, True, (4, 4, 4, 4), "smooth"),
+    (3, 7, 2, True, (3, 2, 4, 3), "rough"),
+    (61, 2, 1, True, (2, 3, 3, 4), "rough"),
 ]
 
 
