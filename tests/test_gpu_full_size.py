@@ -10,6 +10,9 @@
   40 accepted steps (mass conservation in FP64 totals, mirror symmetry of the
   circular dam break, positivity, agreement of the device-controlled step
   sequence with the host-side controller's arithmetic).
+* 16384^2, the largest size tested (byte offsets far past 2^32): the FP64
+  fused engine bit for bit equal to the FP64 staged engine after 10 steps,
+  FP32 close to both, mass conserved.
 """
 from __future__ import annotations
 
@@ -154,3 +157,45 @@ def test_fp32_8192_properties():
         dt = dt * 1.2
         t = t + dt
         assert dt_next == dt and time == t
+
+
+def test_max_size_16384_fused_equals_staged():
+    """16384^2, four times config 2's cells (byte offsets of the FP64 pool and
+    of the staged engine's work arrays far past 2^32): 10 accepted steps of
+    the dam break, the FP64 fused engine bit for bit equal to the FP64
+    staged engine (the two share the per-cell arithmetic, not the indexing);
+    the FP32 engine within 1e-5 relative L1 of it in h; mass conserved."""
+    abi = _abi()
+    from paraclaws_amd import ic
+
+    n = 16384
+    q0 = ic.dam_break(n, n)
+    states, masses = {}, {}
+    for name, prec, eng in (("fused", abi.FP64, abi.FUSED), ("staged", abi.FP64, abi.STAGED),
+                            ("fp32", abi.FP32, abi.FUSED)):
+        _say(f"16384^2 dam break, 10 accepted steps: {name}")
+        with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=prec, engine=eng,
+                        limiter=4) as e:
+            e.set_state(0, q0)
+            e.fill_ghosts()
+            m0 = e.totals(0)[0]
+            e.time_init(**TKW)
+            e.run(steps_per_graph=10, max_accepted=10)
+            st = e.status(0)
+            assert st.status == abi.STATUS_DONE and st.accepted == 10 and st.rejected == 0
+            masses[name] = (m0, e.totals(0)[0])
+            states[name] = e.get_state(0)
+    del q0
+    _say("16384^2: comparing")
+    a, b = states["fused"].view(np.uint64), states["staged"].view(np.uint64)
+    # every frame cell but the 2x2 ghost corners, which the reference never writes
+    assert np.array_equal(a[2:-2], b[2:-2])
+    assert np.array_equal(a[:2, 2:-2], b[:2, 2:-2]) and np.array_equal(a[-2:, 2:-2], b[-2:, 2:-2])
+    del a, b
+    h64 = states["fused"][2:-2, 2:-2, 0]
+    h32 = states["fp32"][2:-2, 2:-2, 0]
+    assert np.isfinite(h32).all()
+    assert np.abs(h32 - h64).sum() <= 1e-5 * np.abs(h64).sum()
+    for name, (a, b) in masses.items():
+        assert abs(b - a) <= (1e-12 if name != "fp32" else 1e-6) * a, (name, a, b)
+    assert np.abs(states["fused"][2:-2, 2:-2, 1]).max() > 0.0
