@@ -119,30 +119,74 @@ void order_tiles(pcw_handle* h) {
   }
 }
 
+// Row bands [y0, y1] covering 1..ny: bands of ty rows, or, when short_ends,
+// first and last bands of about two thirds of ty (short_all: every band).
+void row_bands(int ny, int ty, bool short_ends, bool short_all, std::vector<int2>& out) {
+  out.clear();
+  const int te = std::max(1, (2 * ty + 2) / 3);
+  if (short_all) ty = te;
+  if (!short_ends || short_all || ny <= 2 * te + ty) {
+    const int nb = (ny + ty - 1) / ty;
+    for (int b = 0; b < nb; ++b) {  // equal heights within one row
+      const int y0 = 1 + (int)((long long)ny * b / nb), y1 = (int)((long long)ny * (b + 1) / nb);
+      out.push_back(make_int2(y0, y1));
+    }
+    return;
+  }
+  out.push_back(make_int2(1, te));
+  const int mid = ny - 2 * te, nb = (mid + ty - 1) / ty;
+  for (int b = 0; b < nb; ++b)
+    out.push_back(make_int2(te + 1 + (int)((long long)mid * b / nb), te + (int)((long long)mid * (b + 1) / nb)));
+  out.push_back(make_int2(ny - te + 1, ny));
+}
+
+// Tiles of every instance: strips of kStripOut columns (the last one shifted
+// to end at column nx) times row bands, ordered by band start then column.
+// FP32: tiles on a domain boundary run the kernel's general body, about 1.5x
+// an interior tile's time per row, so they get about two thirds of the rows
+// (every band of the two edge strips, the first and last band of the others):
+// a launch then does not wait on its boundary tiles.
+// number of tiles build_tiles makes for tile height ty
+long long count_tiles(const pcw_handle* h, int ty) {
+  const bool shorten = h->p.precision == PCW_FP32;
+  std::vector<int2> inner, edge;
+  long long t = 0;
+  for (const auto& I : h->hinst) {
+    const int nsx = (I.nx + kStripOut - 1) / kStripOut;
+    row_bands(I.ny, ty, shorten, false, inner);
+    row_bands(I.ny, ty, shorten, shorten, edge);
+    const int nedge = std::min(nsx, 2);
+    t += (long long)(nsx - nedge) * (long long)inner.size() + (long long)nedge * (long long)edge.size();
+  }
+  return t;
+}
+
 void build_tiles(pcw_handle* h, int ty) {
   h->htiles.clear();
+  const bool shorten = h->p.precision == PCW_FP32;
+  std::vector<int2> inner, edge;
   for (int k = 0; k < h->n; ++k) {
     KInst& I = h->hinst[k];
     I.tile0 = (int)h->htiles.size();
-    std::vector<int> xs, y0s;
+    std::vector<int> xs;
     const int nsx = (I.nx + kStripOut - 1) / kStripOut;
     for (int s = 0; s < nsx; ++s) {
       int x0 = 1 + kStripOut * s;
       if (x0 + kStripOut - 1 > I.nx) x0 = std::max(1, I.nx - kStripOut + 1);
       xs.push_back(x0);
     }
-    const int nsy = (I.ny + ty - 1) / ty;
-    int count = 0;
-    for (int b = 0; b < nsy; ++b) {
-      int y0 = 1 + ty * b;
-      if (y0 + ty - 1 > I.ny) y0 = std::max(1, I.ny - ty + 1);
-      int y1 = std::min(y0 + ty - 1, I.ny);
-      for (int x0 : xs) {
-        h->htiles.push_back(make_int4(k, x0, y0, y1));
-        ++count;
-      }
+    row_bands(I.ny, ty, shorten, false, inner);
+    row_bands(I.ny, ty, shorten, shorten, edge);
+    std::vector<int4> t;
+    for (int s = 0; s < nsx; ++s) {
+      const bool is_edge = s == 0 || s == nsx - 1;
+      for (const int2& b : is_edge ? edge : inner) t.push_back(make_int4(k, xs[s], b.x, b.y));
     }
-    I.ntiles = count;
+    std::stable_sort(t.begin(), t.end(), [](const int4& a, const int4& b) {
+      return a.z != b.z ? a.z < b.z : a.y < b.y;
+    });
+    h->htiles.insert(h->htiles.end(), t.begin(), t.end());
+    I.ntiles = (int)t.size();
   }
   h->ntiles = (int)h->htiles.size();
   order_tiles(h);
@@ -893,12 +937,7 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
                                             any_src, 64 * wv);
     if (bpc <= 0) continue;
     const long long slots = (long long)bpc * wv * cu;  // resident waves
-    auto ntiles = [&](int tr) {
-      long long t = 0;
-      for (const auto& d : h->dom)
-        t += (long long)((d.nx + kStripOut - 1) / kStripOut) * ((d.ny + tr - 1) / tr);
-      return t;
-    };
+    auto ntiles = [&](int tr) { return count_tiles(h, tr); };
     for (int k = 1; k <= 6; ++k) {
       int lo = 8, hi = std::max(8, h->max_ny);
       if (ntiles(hi) > k * slots) continue;
