@@ -376,6 +376,8 @@ bool metrics_are_unit(const KInst& I) {
     if (gamma_eta_exact(M, i) != 1.0) return false;
     const double x1 = corner_x1(M, i), w = (x1 + M.dxi) - x1;
     if (std::find(widths.begin(), widths.end(), w) == widths.end()) {
+      // a uniform grid rounds to one or a few widths; many widths mean metrics
+      // that are not all 1 anyway (gamma_eta = width / dxi), so stop early
       if (widths.size() == 16) return false;
       widths.push_back(w);
       cols.push_back(i);
