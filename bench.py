@@ -301,8 +301,9 @@ def main():
                     help="run a plain device copy of MIB MiB (FETCH/WRITE counter scale)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(how the domain would shard is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py runs on one GPU only: the engine has no multi-GPU path "
+              "(one domain per GPU; a sharded domain would exchange 2-row halo strips "
+              "in place of the ghost fill).", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:
