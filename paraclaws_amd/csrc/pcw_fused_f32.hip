@@ -178,21 +178,25 @@ __device__ __forceinline__ void cell(float h, float hu, float hv, float g, float
 //   w1 = qs - qL = (df - s2 dq - eps qL) / D,  w2 = qR - qs = -(df + s2 dq - eps qR) / D.
 // The eps q terms (below FP32 resolution next to the others, and exactly
 // what the reference leaks into waves of a quiescent state) are dropped, so
-// a state at rest has exactly zero waves.  Waves and speed are zero when
-// s2 <= eps, through a zero reciprocal: no branch.
+// a state at rest has exactly zero waves, and then w1 + w2 = dq * 2 s2 /
+// (2 s2 - eps), which is dq to FP32 resolution: w2 = dq - w1.  Waves are
+// zero when s2 <= eps, through a zero reciprocal: no branch.  The speed is
+// returned as it is: at s2 <= eps (1e-10) its waves are zero, and its share
+// of the CFL maximum is below any live interface's.
 __device__ __forceinline__ float hll(const float qL[3], const float fL[3], float sL,
                                      const float qR[3], const float fR[3], float sR, float eps,
                                      float w1[3], float w2[3]) {
   const float s2 = smax2(sL, sR);
   const bool live = s2 > eps;
   const float rD = live ? rcp(fmaf(-2.0f, s2, eps)) : 0.0f;
+  const float m = live ? 1.0f : 0.0f;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const float dq = qR[k] - qL[k], df = fR[k] - fL[k];
     w1[k] = fmaf(-s2, dq, df) * rD;
-    w2[k] = -fmaf(s2, dq, df) * rD;
+    w2[k] = fmaf(dq, m, -w1[k]);
   }
-  return live ? s2 : 0.0f;
+  return s2;
 }
 
 // Half the limiter function, phi(theta) / 2, of one wave
