@@ -76,13 +76,15 @@ def test_mass_conserved_with_walls(oracle, ho, use_lim):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("source", [1, 3])
-def test_gpu_fp64_sources_vs_oracle(oracle, source):
+@pytest.mark.parametrize("nx,ny", [(90, 70), (64, 32)], ids=["computed-metrics", "unit-metrics"])
+def test_gpu_fp64_sources_vs_oracle(oracle, source, nx, ny):
     """FP64 fused engine with bathymetry (and friction) + wet/dry vs the
-    oracle.  Bathymetry alone is bit-exact; friction uses cbrt, which is not
-    correctly rounded on either side, so that case is held to 1e-13."""
+    oracle, on a grid whose metrics the kernel computes and on one whose
+    metrics are all exactly 1.  Bathymetry alone is bit-exact; friction uses
+    cbrt, which is not correctly rounded on either side, so that case is held
+    to 1e-13."""
     from paraclaws_amd import abi, ic
 
-    nx, ny = 90, 70
     b = ic.gaussian_bathymetry(nx, ny, seed=3)
     q = ic.still_water_with_step(b, nx, dry_tol=1e-6)
     q[..., 0] = np.maximum(q[..., 0], 0.0)
