@@ -51,6 +51,9 @@ CASES = [
     (2, 2, 4, True, (4, 4, 4, 4), "smooth"),
     (3, 7, 2, True, (3, 2, 4, 3), "rough"),
     (61, 2, 1, True, (2, 3, 3, 4), "rough"),
+    # more grids whose metrics are all exactly 1 (the FP64 unit-metric kernel)
+    (128, 32, 3, True, (3, 4, 2, 3), "rough"),
+    (16, 64, 2, False, (4, 2, 3, 4), "smooth"),
 ]
 
 
