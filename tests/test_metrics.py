@@ -98,28 +98,34 @@ def test_gpu_fp64_mapped_bitwise(oracle, engine, mapping):
 
 @pytest.mark.gpu
 def test_gpu_mixed_batch_mapped_and_plain(oracle):
-    """One handle with a mapped instance and an identity-map instance (two
-    kernel variants per step) equals each instance run alone."""
+    """One handle with a mapped instance, an identity-map instance whose
+    metrics the FP64 kernel computes and one whose metrics are exactly 1
+    (three kernel variants per FP64 step, two per FP32 step) equals each
+    instance run alone."""
     from paraclaws_amd import abi
 
     n = 48
     cfg_m, (k, gx, ge, _, _) = _cfg(oracle, n, n, STRETCH, limiter=4, bc=(4, 4, 4, 4))
     qa = oracle.dam_break(oracle.Config(nx=n, ny=n))
     qb = oracle.dam_break(oracle.Config(nx=n, ny=n), h_in=3.0)
+    qc = oracle.dam_break(oracle.Config(nx=64, ny=64), h_in=2.5)
+    sizes = [n, n, 64]
     tkw = dict(t_final=0.05, dt0=1e-4, n_snapshots=2)
     for prec in (abi.FP64, abi.FP32):
         print(f"gpu mixed batch precision={prec}", flush=True)
-        with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4)), abi.Domain(n, n, bc=(4, 4, 4, 4))],
+        with abi.Engine([abi.Domain(s_, s_, bc=(4, 4, 4, 4)) for s_ in sizes],
                         precision=prec, limiter=4) as e:
             e.set_state(0, qa)
             e.set_state(1, qb)
+            e.set_state(2, qc)
             e.set_metrics(0, k, gx, ge)
             e.fill_ghosts()
             e.time_init(**tkw)
             e.run(steps_per_graph=8)
-            both = [e.get_state(0), e.get_state(1)]
-        for idx, (q0, met) in enumerate(((qa, True), (qb, False))):
-            with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=prec, limiter=4) as e1:
+            every = [e.get_state(i) for i in range(3)]
+        for idx, (q0, met) in enumerate(((qa, True), (qb, False), (qc, False))):
+            with abi.Engine([abi.Domain(sizes[idx], sizes[idx], bc=(4, 4, 4, 4))],
+                            precision=prec, limiter=4) as e1:
                 e1.set_state(0, q0)
                 if met:
                     e1.set_metrics(0, k, gx, ge)
@@ -127,7 +133,7 @@ def test_gpu_mixed_batch_mapped_and_plain(oracle):
                 e1.time_init(**tkw)
                 e1.run(steps_per_graph=8)
                 solo = e1.get_state(0)
-            assert np.array_equal(both[idx], solo), (prec, idx)
+            assert np.array_equal(every[idx], solo), (prec, idx)
 
 
 @pytest.mark.gpu
