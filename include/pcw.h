@@ -18,14 +18,14 @@
  *   pcw_fill_ghosts      -> update_rank_boundaries (:17-56) on one rank
  *   pcw_source           -> integrate_source_term (MOD_solve.f90:295-336)
  *   pcw_set_state/get_state -> solution%vars(3,ilow-2:ihigh+2,jlow-2:jhigh+2)
- *                          (src/domain/SMOD_domain_allocation.f90:231)
+ *                          (src/domain/SMOD_domain_allocation.f90:84-100)
  *   pcw_run              -> the time loop of src/main.f90:193-283 with the
  *                          controller of src/time_control/MOD_time_control.f90
  *                          replicated on the device
  *
  * Return codes: 0 ok, 1 step rejected (solve_dimensional_split's ierr = -1),
  * negative = fatal (the host maps it to abort_program,
- * src/utility/MOD_error_handling.f90:122-135); pcw_last_error() describes it.
+ * src/utility/MOD_error_handling.f90:37-50); pcw_last_error() describes it.
  */
 #ifndef PCW_H
 #define PCW_H
@@ -146,6 +146,9 @@ int pcw_set_metrics(pcw_handle* h, int32_t inst, const double* kappa, const doub
  * sweep is (smax*dt)/delta, exactly max(lambda*dt/delta) of MOD_solve.f90:81.
  * The host then calls pcw_accept (keep) or pcw_reject (roll back). */
 int pcw_step(pcw_handle* h, const double* dt, double* smax);
+/* pcw_accept / pcw_reject apply to the instances pcw_step advanced: those
+ * whose status is RUNNING (every instance after pcw_create; a device-
+ * controlled run leaves finished ones DONE until pcw_time_init). */
 int pcw_accept(pcw_handle* h);
 int pcw_reject(pcw_handle* h);
 /* Single sweeps in place on the current state (staged engine semantics). */
@@ -162,7 +165,11 @@ int pcw_read_log(pcw_handle* h, int32_t inst, pcw_step_record* out, int32_t cap,
                  int32_t* n);
 /* Runs graph-captured blocks of steps_per_graph attempts until every
  * instance is done or max_attempts attempts were made (0 = unbounded).
- * max_accepted > 0 also stops an instance after that many accepted steps. */
+ * max_accepted > 0 also stops an instance after that many accepted steps;
+ * a block is then shortened to the accepted steps the running instances
+ * still need, so that no launch of it only finds finished instances.  An
+ * instance that an earlier run stopped at its max_accepted continues when
+ * the limit is raised (or 0). */
 int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts,
             int64_t max_accepted);
 int pcw_get_status(pcw_handle* h, int32_t inst, pcw_status* out);
@@ -173,8 +180,10 @@ int pcw_totals(pcw_handle* h, int32_t inst, double out[3]);
 /* Device time (ms) of the last pcw_run / pcw_step, measured with HIP events
  * on the handle's stream. */
 int pcw_last_elapsed_ms(pcw_handle* h, double* ms);
-/* Average duration (ms) of one fused step kernel over the last pcw_run,
- * from HIP events bracketing each graph launch divided by its steps. */
+/* Average duration (ms) of one attempt over the last pcw_run: HIP events
+ * bracket each graph launch on the handle's stream; the average is taken over
+ * the graphs whose every launch made an attempt (else over all attempts).
+ * launches: step launches of the run. */
 int pcw_kernel_stats(pcw_handle* h, double* avg_step_ms, int64_t* launches);
 /* Autotuner: times candidate (tile_rows, waves_per_block) on the handle's
  * current state and keeps the fastest; writes the choice. */

@@ -51,7 +51,7 @@ static inline long F(const frame_t* f, int i, int j) {
 /* ------------------------------------------------------------ mesh metrics
  * src/domain/SMOD_mesh_manipulation.f90:28-51 (coordinates) and :147-173
  * (length ratios, shoelace area, capacity) for the identity map
- * (src/user_subroutines/SMOD_coordinate_mapping.f90:295-296). */
+ * (src/user_subroutines/SMOD_coordinate_mapping.f90:35-43). */
 static double d_xi_of(const orc_cfg* c) { return (c->xi_max - c->xi_min) / c->nx; }
 static double d_eta_of(const orc_cfg* c) { return (c->eta_max - c->eta_min) / c->ny; }
 
@@ -142,7 +142,7 @@ int orc_init_dam_break(const orc_cfg* c, double* vars, double h_in,
 
 /* ---------------------------------------------------------- ghost fill
  * src/user_subroutines/SMOD_boundary_conditions.f90:58-213 (BC 2 and 3),
- * deriv_2p src/utility/SMOD_derivatives.f90:225.  Corners are never filled.
+ * deriv_2p src/utility/SMOD_derivatives.f90:5-11.  Corners are never filled.
  * BC 4 (reflective wall: mirror the first two interior cells, negate the
  * normal momentum) is this build's addition. */
 static inline double deriv_2p(double x1, double x2, double y1, double y2) {
@@ -252,7 +252,7 @@ static inline int phys_x(const orc_cfg* c, const double q[3], double* s,
   if (isnan(J21) || isnan(J22) || isnan(J31) || isnan(J32) || isnan(J33)) return 1;
   /* dgeev returns u +- i*sqrt(-g h) for g h < 0; the reference stops only
    * when that imaginary part exceeds ud_precision
-   * (src/utility/SMOD_eigenvalue_problem.f90:258-268) */
+   * (src/utility/SMOD_eigenvalue_problem.f90:72-83) */
   double gh = g * h, cc = 0.0;
   if (gh < 0.0) { if (sqrt(-gh) > c->eps) return 2; } else cc = sqrt(gh);
   *s = fabs(J33) + cc;
@@ -289,7 +289,7 @@ static inline int phys_y(const orc_cfg* c, const double q[3], double* s,
 }
 
 /* --------------------------------------------------------- work arrays
- * src/solver/MOD_work_arrays.f90:250-258; zeroed after every sweep (:267-298). */
+ * src/solver/MOD_work_arrays.f90:14-70; zeroed after every sweep (:78-109). */
 typedef struct {
   double *lambda, *waves, *Am, *Ap, *wl, *wr, *f;
   long n;
@@ -328,7 +328,7 @@ static void work_zero(work_t* w) {
 #define WAV(w, k, me, mw) (w)->waves[6 * (k) + 3 * (mw) + (me)]
 
 /* ------------------------------------------------------------- HLL
- * src/solver/SMOD_approximate_Riemann_solvers.f90:169-224 for the interface
+ * src/solver/SMOD_approximate_Riemann_solvers.f90:15-102 for the interface
  * between cell L (i-1 or j-1) and R (i or j). */
 static inline int hll(const orc_cfg* c, int ixy, const double* qL,
                       const double* qR, work_t* w, long k) {
@@ -677,17 +677,26 @@ int orc_solve_dimensional_split(const orc_cfg* c, orc_tc* tc, double* vars) {
 /* --------------------------------------------------------- source term
  * integrate_source_term, src/solver/MOD_solve.f90:295-336: forward Euler
  * q += phi*dt over the interior.  The reference's phi is 0
- * (src/user_subroutines/SMOD_source_function.f90:255), for which q is left
- * unchanged.  The other kinds are this build's physics:
+ * (src/user_subroutines/SMOD_source_function.f90:15-28); q + 0*dt leaves
+ * every value as it is except an interior -0.0, which becomes +0.0.  The
+ * other kinds are this build's physics:
  *   bathymetry  phi_hu = -g h (b(i+1)-b(i-1))/dxi, phi_hv likewise in eta
  *               (pressure term g h^2 of the reference flux, so the balancing
  *               slope term carries 2 g h db/dx),
  *   friction    semi-implicit Manning: hu /= 1 + dt g n^2 |u| / h^(4/3),
  *   wet/dry     cells with h <= dry_tol get hu = hv = 0 and h = max(h, 0). */
 int orc_source(const orc_cfg* c, double* vars, const double* bathy, double dt) {
-  if (c->source == ORC_SRC_NONE && !(c->dry_tol > 0.0)) return 0;
   frame_t fr = frame_of(c);
   const frame_t* f = &fr;
+  if (c->source == ORC_SRC_NONE && !(c->dry_tol > 0.0)) {
+    /* the reference's q = q + phi*dt with phi = 0 (MOD_solve.f90:319-327) */
+    const double zero_dt = 0.0 * dt;
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= c->ny; ++j)
+      for (int i = 1; i <= c->nx; ++i)
+        for (int me = 0; me < 3; ++me) Q(vars, f, me, i, j) = Q(vars, f, me, i, j) + zero_dt;
+    return 0;
+  }
   const double g = c->grav, dxi = d_xi_of(c), deta = d_eta_of(c);
   const int use_b = (c->source == ORC_SRC_BATHY || c->source == ORC_SRC_BATHY_FRICTION);
   const int use_fr = (c->source == ORC_SRC_FRICTION || c->source == ORC_SRC_BATHY_FRICTION);
@@ -718,8 +727,11 @@ int orc_source(const orc_cfg* c, double* vars, const double* bathy, double dt) {
         nhu = nhu / den;
         nhv = nhv / den;
       }
-      Q(vars, f, 1, i, j) = nhu;
-      Q(vars, f, 2, i, j) = nhv;
+      /* every kind ends with the reference's + 0*dt, so that a zero slope or
+       * a zero Manning coefficient gives the phi = 0 step's bits */
+      Q(vars, f, 0, i, j) = h + 0.0;
+      Q(vars, f, 1, i, j) = nhu + 0.0;
+      Q(vars, f, 2, i, j) = nhv + 0.0;
     }
   return 0;
 }

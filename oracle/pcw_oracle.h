@@ -9,8 +9,8 @@
  * Layout: every field is a Fortran-style frame vars(3, -1:nx+2, -1:ny+2)
  * (equation index fastest, then i, then j; column-major), exactly the
  * reference's solution%vars with ilow=jlow=1 and num_ghost=2
- * (src/domain/SMOD_domain_allocation.f90:231).  Work arrays follow
- * src/solver/MOD_work_arrays.f90:250-258.
+ * (src/domain/SMOD_domain_allocation.f90:84-100).  Work arrays follow
+ * src/solver/MOD_work_arrays.f90:14-70.
  */
 #ifndef PCW_ORACLE_H
 #define PCW_ORACLE_H
@@ -28,7 +28,7 @@ extern "C" {
 #define ORC_BC_CONSTANT_GRADIENT 3
 #define ORC_BC_REFLECTIVE 4
 
-/* Source kinds.  0 is the reference's phi = 0 (SMOD_source_function.f90:255);
+/* Source kinds.  0 is the reference's phi = 0 (SMOD_source_function.f90:15-28);
  * the others are physics of this build with no reference counterpart. */
 #define ORC_SRC_NONE 0
 #define ORC_SRC_BATHY 1      /* -g h grad(b) with wet/dry cut-off            */

@@ -49,9 +49,9 @@ struct pcw_handle {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tile_rows = 64, wpb = 2;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  int gsteps = 0;
+  // captured step graphs by length (attempts per launch); a run uses its
+  // full length and, for the attempts left before max_accepted, a shorter one
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;
   std::string err;
   double last_ms = 0.0, avg_step_ms = 0.0;
   long long launches = 0;
@@ -216,11 +216,8 @@ int push_states(pcw_handle* h) {
 }
 
 void drop_graph(pcw_handle* h) {
-  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
-  h->gexec = nullptr;
-  h->graph = nullptr;
-  h->gsteps = 0;
+  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);
+  h->graphs.clear();
 }
 
 int limiter_code(const pcw_params& p) { return (p.higher_order && p.use_limiter) ? p.limiter : 0; }
@@ -297,16 +294,14 @@ int enqueue_staged_sweeps(pcw_handle* h, int backup) {
   return PCW_OK;
 }
 
+// source term (every instance: without a source kind it is the reference's
+// q + 0*dt) and ghost fill of the staged engine's attempt
 template <typename T>
 int enqueue_source_ghosts(pcw_handle* h) {
-  hipError_t e;
-  bool any_src = h->p.dry_tol > 0.0;
-  for (const auto& d : h->dom) any_src = any_src || d.source != PCW_SRC_NONE;
-  if (any_src) {
-    e = launch_source<T>((T*)h->pool, (const T*)h->bpool, h->dinst, h->dst, h->n, h->max_nx,
-                         h->max_ny, h->p.dry_tol > 0.0, h->p.grav, h->p.dry_tol, h->stream);
-    if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
-  }
+  hipError_t e = launch_source<T>((T*)h->pool, (const T*)h->bpool, h->dinst, h->dst, h->n,
+                                  h->max_nx, h->max_ny, h->p.dry_tol > 0.0, h->p.grav,
+                                  h->p.dry_tol, h->stream);
+  if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
   e = launch_ghosts<T>((T*)h->pool, h->dinst, h->dst, h->n, h->max_side, kGhostsCur, h->stream);
   if (e != hipSuccess) return fail(h, PCW_ERR_HIP, hipGetErrorString(e));
   return PCW_OK;
@@ -389,6 +384,39 @@ bool metrics_are_unit(const KInst& I) {
       if (kappa_exact(M, i, j) != 1.0) return false;
   }
   return true;
+}
+
+// The instantiated graph of `len` device-controlled attempts, captured on
+// first use.  At most a few lengths are kept (a run uses two).
+int graph_of(pcw_handle* h, int len, hipGraphExec_t* out) {
+  for (auto& g : h->graphs)
+    if (g.first == len) {
+      *out = g.second;
+      return PCW_OK;
+    }
+  if (h->graphs.size() >= 4) {
+    (void)hipGraphExecDestroy(h->graphs.front().second);
+    h->graphs.erase(h->graphs.begin());
+  }
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  for (int s = 0; s < len; ++s) {
+    int rc = enqueue_attempt(h);
+    if (rc) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(h->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+  }
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamEndCapture(h->stream, &graph));
+  hipGraphExec_t gx = nullptr;
+  hipError_t e = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(h, PCW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  h->graphs.push_back({len, gx});
+  *out = gx;
+  return PCW_OK;
 }
 
 }  // namespace
@@ -811,54 +839,70 @@ int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts, int64_
   int rc = pull_states(h);
   if (rc) return rc;
   for (int k = 0; k < h->n; ++k) {
-    if (!h->hst[k].controlled) return fail(h, PCW_ERR_ARG, "call pcw_time_init before pcw_run");
-    h->hst[k].max_accepted = max_accepted;
+    InstState& S = h->hst[k];
+    if (!S.controlled) return fail(h, PCW_ERR_ARG, "call pcw_time_init before pcw_run");
+    S.max_accepted = max_accepted;
+    // an instance that an earlier run stopped at its accepted-step limit
+    // (not at the final time, not by an error) continues under a higher limit
+    if (S.status == PCW_STATUS_DONE && !S.tc.ended && S.tc.time < S.tc.time_final &&
+        (max_accepted == 0 || S.accepted < max_accepted))
+      S.status = PCW_STATUS_RUNNING;
   }
   rc = push_states(h);
   if (rc) return rc;
-  if (!h->gexec || h->gsteps != steps_per_graph) {
-    drop_graph(h);
-    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < steps_per_graph; ++s) {
-      rc = enqueue_attempt(h);
-      if (rc) {
-        hipGraph_t g;
-        (void)hipStreamEndCapture(h->stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-    }
-    HIPCHK(hipStreamEndCapture(h->stream, &h->graph));
-    HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
-    h->gsteps = steps_per_graph;
-  }
-  long long attempts = 0;
-  double graph_ms = 0.0;
   long long before = 0;
   for (int k = 0; k < h->n; ++k)
     before = std::max(before, h->hst[k].accepted + h->hst[k].rejected);
+  long long attempts = 0, work_launches = 0;
+  double graph_ms = 0.0, work_ms = 0.0;
   h->launches = 0;
   for (;;) {
+    bool running = false;
+    for (int k = 0; k < h->n; ++k) running = running || h->hst[k].status == PCW_STATUS_RUNNING;
+    if (!running) break;
+    if (max_attempts > 0 && attempts >= max_attempts) break;
+    // attempts in this launch: the full graph, or fewer when every running
+    // instance needs fewer accepted steps to reach max_accepted (a launch
+    // past that point would only dispatch tiles that exit at once)
+    int len = steps_per_graph;
+    if (max_accepted > 0) {
+      long long need = 0;
+      for (int k = 0; k < h->n; ++k)
+        if (h->hst[k].status == PCW_STATUS_RUNNING)
+          need = std::max(need, max_accepted - h->hst[k].accepted);
+      len = (int)std::max(1ll, std::min<long long>(len, need));
+    }
+    if (max_attempts > 0) len = (int)std::max(1ll, std::min<long long>(len, max_attempts - attempts));
+    hipGraphExec_t gx = nullptr;
+    rc = graph_of(h, len, &gx);
+    if (rc) return rc;
+    long long a0 = 0;
+    for (int k = 0; k < h->n; ++k) a0 = std::max(a0, h->hst[k].accepted + h->hst[k].rejected);
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+    HIPCHK(hipGraphLaunch(gx, h->stream));
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     rc = pull_states(h);
     if (rc) return rc;
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     graph_ms += ms;
-    h->launches += steps_per_graph;
-    attempts += steps_per_graph;
-    bool running = false;
-    for (int k = 0; k < h->n; ++k) running = running || h->hst[k].status == PCW_STATUS_RUNNING;
-    if (!running) break;
-    if (max_attempts > 0 && attempts >= max_attempts) break;
+    h->launches += len;
+    attempts += len;
+    long long a1 = 0;
+    for (int k = 0; k < h->n; ++k) a1 = std::max(a1, h->hst[k].accepted + h->hst[k].rejected);
+    if (a1 - a0 == len) {  // every launch of this graph did an attempt
+      work_ms += ms;
+      work_launches += len;
+    }
   }
   long long after = 0;
   for (int k = 0; k < h->n; ++k)
     after = std::max(after, h->hst[k].accepted + h->hst[k].rejected);
   h->last_ms = graph_ms;
-  h->avg_step_ms = (after > before) ? graph_ms / (double)(after - before) : 0.0;
+  // average launch time over the graphs whose launches all did work; when a
+  // run ends inside every graph, over the attempts made
+  h->avg_step_ms = work_launches > 0 ? work_ms / (double)work_launches
+                   : (after > before) ? graph_ms / (double)(after - before) : 0.0;
   return check_errors(h);
 }
 

@@ -167,7 +167,7 @@ __device__ __forceinline__ CellQ<double> cell_exact(double h, double hu, double 
 }
 
 // ----------------------------------------------------------------- waves
-// HLL of SMOD_approximate_Riemann_solvers.f90:189-224 for L | R.
+// HLL of SMOD_approximate_Riemann_solvers.f90:60-101 for L | R.
 template <typename T>
 struct Waves {
   T s2;             // lambda(2) = s2, lambda(1) = -s2 (0,0 when skipped)
@@ -294,9 +294,24 @@ __device__ __forceinline__ T ghost_value(int bc, T near, T far, int k, bool norm
   return normal ? -v : v;
 }
 
+// The reference's integrate_source_term with its source_function phi = 0
+// (src/solver/MOD_solve.f90:319-327, SMOD_source_function.f90:15-28): q + 0*dt
+// on every interior cell and field.  The sum is q itself except that an
+// interior -0.0 becomes +0.0 (round to nearest); it is kept, not folded away,
+// so that signs of zero follow the reference too.
+template <typename T>
+__device__ __forceinline__ void reference_source(T& h, T& hu, T& hv) {
+  h = h + T(0);
+  hu = hu + T(0);
+  hv = hv + T(0);
+}
+
 // Source term of this build's physics on one cell (no reference counterpart;
 // the reference's source_function returns 0): wet/dry cut-off, bathymetry
 // slope (balancing the reference flux's pressure g h^2), semi-implicit Manning.
+// Every result ends with the reference's + 0*dt (reference_source), so that a
+// zero bathymetry slope or a zero Manning coefficient gives the bits of the
+// reference's phi = 0 step, signs of zero included.
 template <typename T>
 __device__ __forceinline__ void apply_source(T& h, T& hu, T& hv, int kind, T dry, T g, T dxi,
                                              T deta, T dt, T n2, T bxm, T bxp, T bym, T byp) {
@@ -322,6 +337,7 @@ __device__ __forceinline__ void apply_source(T& h, T& hu, T& hv, int kind, T dry
   }
   hu = nhu;
   hv = nhv;
+  reference_source(h, hu, hv);
 }
 
 }  // namespace pcw

@@ -213,7 +213,7 @@ __host__ __device__ inline int tc_attempt(TimeCtl& t, double smax_x, double smax
 
 // Error semantics of one attempt with the device controller.  The reference
 // stops on a NaN Jacobian or complex eigenvalues of a sweep it runs
-// (SMOD_jakobian.f90:55-70, SMOD_eigenvalue_problem.f90:258-268), but it skips
+// (SMOD_jakobian.f90:55-70, SMOD_eigenvalue_problem.f90:72-83), but it skips
 // the eta sweep when the xi sweep alone already rejects the step
 // (MOD_solve.f90:109-120).  Returns 1 when the instance stopped with an
 // error; else *ok is the accept decision (tc_attempt).

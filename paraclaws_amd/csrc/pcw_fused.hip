@@ -243,6 +243,8 @@ __device__ __forceinline__ void fused_step(const FusedArgs<double>& a) {
         T bxm = from_left(b0), bxp = from_right(b0);
         apply_source<T>(qn[0], qn[1], qn[2], src, dry, g, (T)I.dxi, (T)I.deta, (T)dt, n2, bxm,
                         bxp, bm, bnew);
+      } else {
+        reference_source<T>(qn[0], qn[1], qn[2]);
       }
       const long long o = (long long)(j + 1) * pitch + (i + 1);
       if (out_lane) {

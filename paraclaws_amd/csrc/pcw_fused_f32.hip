@@ -27,11 +27,11 @@
 //   * Divisions are reciprocals, square roots are v_sqrt_f32 (1 ulp).
 //   * A cell with h <= 0 (or NaN) in a sweep's range is reported like the
 //     reference's NaN-in-Jacobian / complex-eigenvalue stop
-//     (SMOD_jakobian.f90:55-58, SMOD_eigenvalue_problem.f90:258-268), unless
+//     (SMOD_jakobian.f90:55-58, SMOD_eigenvalue_problem.f90:72-83), unless
 //     the wet/dry cut-off is enabled.
 //
 // Reference stages fused here (src/solver/MOD_solve.f90:23-175 and callees):
-// wave_decomposition/HLL (SMOD_approximate_Riemann_solvers.f90:189-224),
+// wave_decomposition/HLL (SMOD_approximate_Riemann_solvers.f90:60-101),
 // split_flux, integrate_flux_difference, the limiter
 // (MOD_flux_correction.f90:54-123), second_order_flux_correction,
 // integrator_flux_correction_increment, the CFL maximum (as the maximum wave

@@ -276,9 +276,16 @@ __global__ void k_source(T* pool, const T* bpool, const KInst* inst, InstState* 
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
   if (i > I.nx || j > I.ny) return;
-  if (I.source == PCW_SRC_NONE && !wetdry) return;
   T* q = pool + I.off[S->cur];
   const long long pl = I.plane, k = fidx(I, i, j);
+  if (I.source == PCW_SRC_NONE && !wetdry) {  // the reference's q + 0*dt
+    T h = q[k], hu = q[pl + k], hv = q[2 * pl + k];
+    reference_source<T>(h, hu, hv);
+    q[k] = h;
+    q[pl + k] = hu;
+    q[2 * pl + k] = hv;
+    return;
+  }
   T bxm = 0, bxp = 0, bym = 0, byp = 0;
   if (I.boff >= 0) {
     const T* b = bpool + I.boff;
@@ -391,10 +398,12 @@ __global__ void k_set_dt(InstState* st, const double* dt, int n, int controlled)
   st[k].controlled = controlled;
 }
 
+// pcw_accept of the fused engines: the step kernels wrote the pending buffer
+// of the running instances only, so only those switch to it
 __global__ void k_flip(InstState* st, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  st[k].cur ^= 1;
+  if (st[k].status == PCW_STATUS_RUNNING) st[k].cur ^= 1;
 }
 
 // ------------------------------------------------------------- launchers

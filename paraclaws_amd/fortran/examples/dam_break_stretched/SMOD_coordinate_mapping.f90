@@ -1,5 +1,5 @@
 ! Stretched Cartesian map x = 4 xi, y = 2 eta (the reference's
-! examples/2D_dam_break/cartesian_stretching/SMOD_coordinate_mapping.f90:334-342);
+! examples/2D_dam_break/cartesian_stretching/SMOD_coordinate_mapping.f90:34-42);
 ! replaces the base identity map, so the host uploads the capacity (8) and
 ! length ratios (2 on xi faces, 4 on eta faces) to the engine.
 submodule(MOD_domain) SMOD_coordinate_mapping

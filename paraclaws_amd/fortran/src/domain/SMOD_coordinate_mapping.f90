@@ -1,5 +1,5 @@
 ! Identity map from computational to physical coordinates
-! (src/user_subroutines/SMOD_coordinate_mapping.f90:290-298).  The engine's
+! (src/user_subroutines/SMOD_coordinate_mapping.f90:35-43).  The engine's
 ! metrics assume this map; a USER_DIR override of it is reported at start-up.
 submodule(MOD_domain) SMOD_coordinate_mapping
     implicit none

@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 
 
 class EndProgram(RuntimeError):
-    """end_program (src/utility/MOD_error_handling.f90:105-120)."""
+    """end_program (src/utility/MOD_error_handling.f90:20-35)."""
 
 
 @dataclass

@@ -4,9 +4,9 @@ to pin the oracle.
 It follows the reference's structure rather than the oracle's: per
 interface a 3x3 Jacobian matrix (examples/2D_dam_break/cartesian/
 SMOD_jakobian.f90:44-53), eigenvalues from LAPACK dgeev with right
-eigenvectors requested (SMOD_eigenvalue_problem.f90:242-252, the complex
-check of :258-268), fluxes from BLAS dgemv (SMOD_matrix_operations.f90:299),
-work arrays zeroed after every sweep (MOD_work_arrays.f90:267-298), the serial
+eigenvectors requested (SMOD_eigenvalue_problem.f90:50-66, the complex
+check of :72-83), fluxes from BLAS dgemv (SMOD_matrix_operations.f90:5-23),
+work arrays zeroed after every sweep (MOD_work_arrays.f90:78-109), the serial
 limiter with its wnorm2r carry (MOD_flux_correction.f90:54-123), and the
 split_flux mask form (MOD_flux_calculation.f90:105-120).  Plain Python loops:
 tiny grids only.

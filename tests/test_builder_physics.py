@@ -11,22 +11,72 @@ import pytest
 from tests.helpers import GRAV, sides_mask, smooth_state, stable_dt
 
 
-def test_friction_uniform_state_closed_form(oracle):
-    """Semi-implicit Manning on a uniform state:
-    hu' = hu / (1 + dt g n^2 |u| / h^(4/3))."""
+# A spatially uniform state with Manning friction is the ODE
+#   d|u|/dt = -k |u|^2,  k = g n^2 / h^(4/3)  (h constant, direction of u fixed),
+# whose exact solution is 1/|u|(t) = 1/|u|(0) + k t.  The semi-implicit step
+# hu' = hu / (1 + dt k |u|) gives 1/|u'| = 1/|u| + k dt exactly, so after any
+# sequence of steps 1/|u_K| = 1/|u_0| + k sum(dt_k): the discrete solution is
+# the exact one.  A wrong exponent of h, a dropped g or n^2, or a forward
+# Euler step (hu' = hu - dt k |u| hu, off by O((k |u| dt)^2) per step) fails it.
+FRICTION_H, FRICTION_U, FRICTION_V, FRICTION_N = 0.2, 1.6, -1.2, 0.05
+
+
+def friction_dts(k=24, seed=7):
+    """Uneven steps with k |u| dt up to about 0.2 per step."""
+    return list(np.random.default_rng(seed).uniform(0.02, 0.5, size=k))
+
+
+def friction_exact(t):
+    kf = GRAV * FRICTION_N ** 2 / FRICTION_H ** (4.0 / 3.0)
+    return 1.0 / (1.0 / np.hypot(FRICTION_U, FRICTION_V) + kf * t)
+
+
+def test_friction_uniform_state_exact_ode_solution(oracle):
     nx = ny = 6
-    cfg = oracle.Config(nx=nx, ny=ny, source=2, manning_n=0.03, dry_tol=1e-6)
+    cfg = oracle.Config(nx=nx, ny=ny, source=2, manning_n=FRICTION_N, dry_tol=1e-6)
     q = cfg.frame()
-    h, hu, hv = 0.7, 0.35, -0.14
-    q[...] = [h, hu, hv]
-    dt = 1e-3
-    oracle.source(cfg, q, None, dt)
-    sp = np.hypot(hu / h, hv / h)
-    den = 1.0 + dt * GRAV * 0.03 ** 2 * sp / h ** (4.0 / 3.0)
+    q[...] = [FRICTION_H, FRICTION_H * FRICTION_U, FRICTION_H * FRICTION_V]
+    dts = friction_dts()
+    for dt in dts:
+        oracle.source(cfg, q, None, dt)
     inner = q[2:-2, 2:-2]
-    assert np.allclose(inner[..., 1], hu / den, rtol=1e-14, atol=0)
-    assert np.allclose(inner[..., 2], hv / den, rtol=1e-14, atol=0)
-    assert np.all(inner[..., 0] == h)
+    u, v = inner[..., 1] / inner[..., 0], inner[..., 2] / inner[..., 0]
+    assert np.all(inner[..., 0] == FRICTION_H)
+    assert np.allclose(np.hypot(u, v), friction_exact(sum(dts)), rtol=1e-13, atol=0)
+    assert np.allclose(v / u, FRICTION_V / FRICTION_U, rtol=1e-14, atol=0)
+    # the check separates the semi-implicit step from forward Euler
+    sp = np.hypot(FRICTION_U, FRICTION_V)
+    kf = GRAV * FRICTION_N ** 2 / FRICTION_H ** (4.0 / 3.0)
+    for dt in dts:
+        sp = sp - dt * kf * sp * sp
+    assert abs(sp / friction_exact(sum(dts)) - 1) > 1e-2
+
+
+def test_source_step_is_q_plus_zero_dt(oracle):
+    """The reference's integrate_source_term with phi = 0 computes q + 0*dt
+    (src/solver/MOD_solve.f90:319-327): every interior value is kept except
+    -0.0, which becomes +0.0; ghost cells are not touched.  A zero Manning
+    coefficient and a zero bathymetry give the same bits."""
+    nx, ny = 7, 5
+    q = smooth_state(nx, ny, seed=4)
+    q[3, 4, 1] = -0.0
+    q[5, 2, 2] = -0.0
+    q[4, 6, 1] = 0.0
+    q[0, 3, 1] = -0.0  # a ghost cell keeps its sign
+    outs = []
+    for source, n in ((0, 0.0), (2, 0.0), (1, 0.0), (3, 0.0)):
+        cfg = oracle.Config(nx=nx, ny=ny, source=source, manning_n=n,
+                            dry_tol=1e-6 if source else 0.0)
+        r = q.copy()
+        oracle.source(cfg, r, np.zeros((ny + 4, nx + 4)), 1e-3)
+        outs.append(r)
+    base = outs[0]
+    inner = base[2:-2, 2:-2]
+    assert not np.any(np.signbit(inner) & (inner == 0.0))
+    assert np.array_equal(inner, q[2:-2, 2:-2])  # values (+0.0 == -0.0)
+    assert np.signbit(base[0, 3, 1])
+    for r in outs[1:]:
+        assert np.array_equal(r.view(np.uint64), base.view(np.uint64))
 
 
 def test_bathymetry_linear_slope_closed_form(oracle):
@@ -137,3 +187,128 @@ def test_gpu_fp32_bathymetry_close_to_fp64(oracle):
     assert d.sum() / np.abs(ref[2:-2, 2:-2, 0]).sum() < 1e-4
     assert np.all(got[2:-2, 2:-2, 0] >= 0.0)
     assert tot[0] == pytest.approx(got[2:-2, 2:-2, 0].sum(), rel=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_gpu_fp64_friction_exact_ode_solution(engine):
+    """The FP64 engines' Manning friction on a uniform state against the
+    exact ODE solution (see friction_exact): the standalone source step
+    (pcw_source) after uneven steps, and whole steps (sweeps, source, ghost
+    fill with zero-gradient sides), where the sweeps scale h and hu alike by
+    the reference's epsilon leak, so |u| follows the ODE with the depth
+    h_k of each step: 1/|u_K| = 1/|u_0| + g n^2 sum(dt_k h_k^(-4/3)).  The
+    eta sweep reads ghost rows from before the xi sweep (the reference fills
+    ghosts once per step), so the first and last rows drift from the rest by
+    the leak's difference; that moves in by one row per step, and the rows
+    further in stay uniform bit for bit."""
+    from paraclaws_amd import abi
+
+    nx, ny = 70, 33
+    dom = abi.Domain(nx, ny, bc=(2, 2, 2, 2), source=abi.SRC_FRICTION, manning_n=FRICTION_N)
+    q = dom.frame()
+    q[...] = [FRICTION_H, FRICTION_H * FRICTION_U, FRICTION_H * FRICTION_V]
+    dts = friction_dts()
+    print(f"gpu fp64 friction ode {engine}", flush=True)
+    with abi.Engine([dom], precision=abi.FP64, limiter=4, dry_tol=1e-6,
+                    engine=abi.FUSED if engine == "fused" else abi.STAGED) as e:
+        e.set_state(0, q)
+        for dt in dts:
+            e.source(dt)
+        got = e.get_state(0)[2:-2, 2:-2]
+        u, v = got[..., 1] / got[..., 0], got[..., 2] / got[..., 0]
+        assert np.all(got[..., 0] == FRICTION_H)
+        assert np.allclose(np.hypot(u, v), friction_exact(sum(dts)), rtol=1e-13, atol=0)
+        assert np.allclose(v / u, FRICTION_V / FRICTION_U, rtol=1e-14, atol=0)
+        # whole steps, small enough for the CFL limit of this grid
+        e.set_state(0, q)
+        inv = 1.0 / np.hypot(FRICTION_U, FRICTION_V)
+        for k, dt in enumerate(dts[:12]):
+            e.step(dt * 1e-2)
+            e.accept()
+            s = e.get_state(0)[2:-2, 2:-2]
+            c = s[ny // 2, nx // 2]
+            assert np.all(s[k + 1:ny - k - 1] == c)  # rows away from the edges
+            inv += GRAV * FRICTION_N ** 2 * (dt * 1e-2) / c[0] ** (4.0 / 3.0)
+        sp = np.hypot(c[1] / c[0], c[2] / c[0])
+        assert sp == pytest.approx(1.0 / inv, rel=1e-13)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_gpu_fp64_zero_coefficients_give_the_reference_step(engine):
+    """With a zero Manning coefficient, a zero bathymetry, both, or the
+    wet/dry cut-off on a wet domain, the FP64 engines produce the bits of
+    the reference's phi = 0 run (and of the oracle), signs of zero included:
+    the planted -0.0 momenta become +0.0 in every variant."""
+    from oracle import oracle as O
+    from paraclaws_amd import abi
+
+    nx, ny = 67, 45
+    q = smooth_state(nx, ny, seed=8)
+    q[10, 20, 1] = -0.0
+    q[30, 7, 2] = -0.0
+    cfg = O.Config(nx=nx, ny=ny, bc=(4, 2, 3, 4), limiter=4)
+    O.fill_ghosts(cfg, q)
+    ref = q.copy()
+    ts = O.TimeSettings(t_final=0.05, cfl_target=0.9, dt0=1e-4, n_snap=2)
+    recs, _ = O.run(cfg, ts, ref, max_accepted=15)
+    variants = [(abi.SRC_NONE, 0.0, 0.0), (abi.SRC_FRICTION, 0.0, 0.0),
+                (abi.SRC_BATHY, 0.0, 0.0), (abi.SRC_BATHY_FRICTION, 0.0, 0.0),
+                (abi.SRC_NONE, 0.0, 1e-6)]
+    m = sides_mask(nx, ny)
+    for src, n, dry in variants:
+        print(f"gpu fp64 zero coefficients {engine} source={src} dry={dry}", flush=True)
+        dom = abi.Domain(nx, ny, bc=cfg.bc, source=src, manning_n=n)
+        with abi.Engine([dom], precision=abi.FP64, limiter=4, dry_tol=dry,
+                        engine=abi.FUSED if engine == "fused" else abi.STAGED) as e:
+            e.set_state(0, q)
+            if src in (abi.SRC_BATHY, abi.SRC_BATHY_FRICTION):
+                e.set_bathymetry(0, np.zeros((ny + 4, nx + 4)))
+            e.time_init(t_final=0.05, cfl_target=0.9, dt0=1e-4, n_snapshots=2)
+            e.enable_log(64)
+            e.run(steps_per_graph=5, max_accepted=15)
+            assert e.read_log(0) == recs
+            got = e.get_state(0)
+        assert np.array_equal(got[m].view(np.uint64), ref[m].view(np.uint64))
+        inner = got[2:-2, 2:-2]
+        assert not np.any(np.signbit(inner) & (inner == 0.0))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gpu_lake_at_rest_residual(prec):
+    """Still water h + b = 1 over the Gaussian-sum bathymetry (wet everywhere,
+    b <= 0.8), walls, 50 steps: the residual is reported, not pinned.  The
+    slope term 2 g h db/dx balances the reference flux's g h^2 pressure, but
+    the HLL waves carry the jump of h, not of h + b, so the scheme is not
+    well balanced: at CFL 0.45 a slow spurious flow remains (FP64 oracle:
+    max|h+b-1| = 3.4e-3, max|u| = 0.046 at t = 0.047); at CFL 0.9 the
+    reference scheme's growing modes amplify it (0.36 and 1.07 at t = 0.082)."""
+    from paraclaws_amd import abi, ic
+
+    n = 256
+    b = ic.gaussian_bathymetry(n, n, seed=3)
+    b *= 0.8 / b.max()  # wet everywhere: h >= 0.2
+    q = ic.still_water_with_step(b, n, step=0.0, dry_tol=1e-6)
+    assert q[..., 0].min() > 0.0
+    dom = abi.Domain(n, n, bc=(4, 4, 4, 4), source=abi.SRC_BATHY)
+    res = {}
+    for cfl in (0.45, 0.9):
+        print(f"lake at rest {prec} cfl={cfl}", flush=True)
+        with abi.Engine([dom], precision=abi.FP64 if prec == "fp64" else abi.FP32, limiter=4,
+                        dry_tol=1e-6) as e:
+            e.set_state(0, q)
+            e.set_bathymetry(0, b)
+            e.fill_ghosts()
+            e.time_init(t_final=1.0, cfl_target=cfl, dt0=1e-4, n_snapshots=1)
+            e.run(steps_per_graph=10, max_accepted=50)
+            st = e.status(0)
+            got = e.get_state(0)[2:-2, 2:-2]
+        assert np.isfinite(got).all() and got[..., 0].min() > 0.0
+        eta = got[..., 0] + b[2:-2, 2:-2]
+        vel = np.hypot(got[..., 1], got[..., 2]) / got[..., 0]
+        res[cfl] = (np.abs(eta - 1).max(), vel.max())
+        print(f"  t={st.time:.4e} max|h+b-1|={res[cfl][0]:.3e} max|u|={res[cfl][1]:.3e}",
+              flush=True)
+    assert res[0.45][0] < 5e-3 and res[0.45][1] < 0.06

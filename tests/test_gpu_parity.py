@@ -505,3 +505,61 @@ def test_finished_instance_state_untouched_by_later_runs():
         st2 = e.status(0)
         assert (st2.accepted, st2.rejected) == (st.accepted, st.rejected)
         assert np.array_equal(e.get_state(0), q)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_autotune_mid_run_keeps_the_trajectory(prec):
+    """pcw_autotune times candidate tiles on a handle in the middle of a
+    device-controlled run: the run that follows is the one a fixed-tile
+    handle makes, bit for bit (state, status, time and the step log).  Cell
+    values do not depend on the tiling, and the autotuner restores the
+    controller state it steps from."""
+    abi = _abi()
+    n = 200
+    precision = abi.FP64 if prec == "fp64" else abi.FP32
+    kw = dict(t_final=0.2, dt0=1e-6, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10, n_snapshots=4)
+    out = []
+    for tune in (False, True):
+        _say(f"autotune mid-run {prec} tune={tune}")
+        with abi.Engine([abi.Domain(n, n, bc=(4, 2, 3, 4))], precision=precision, limiter=4,
+                        tile_rows=64, waves_per_block=2) as e:
+            e.set_state(0, smooth_state(n, n, seed=11))
+            e.fill_ghosts()
+            e.time_init(**kw)
+            e.enable_log(400)
+            e.run(steps_per_graph=8, max_accepted=12)
+            if tune:
+                rows, waves = e.autotune(steps=2)
+                assert rows >= 1 and waves in (1, 2, 4)
+            e.run(steps_per_graph=8, max_accepted=40)
+            st = e.status(0)
+            out.append((e.get_state(0), (st.status, st.accepted, st.rejected, st.time, st.dt),
+                        e.read_log(0)))
+    (q0, s0, l0), (q1, s1, l1) = out
+    assert s0 == s1 and s0[1] == 40
+    assert l0 == l1
+    m = sides_mask(n, n)
+    assert np.array_equal(q0[m].view(np.uint64), q1[m].view(np.uint64))
+
+
+def test_host_steps_after_a_finished_run_leave_finished_instances():
+    """Host stepping after a device-controlled run: the step kernels advance
+    the running instances only, and pcw_accept switches those only, so an
+    instance the run left DONE keeps its state (not the stale pending frame)."""
+    abi = _abi()
+    n = 40
+    _say("host steps after a finished run")
+    with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4)), abi.Domain(n, n, bc=(4, 4, 4, 4))],
+                    precision=abi.FP32, limiter=4) as e:
+        for k in range(2):
+            e.set_state(k, smooth_state(n, n, seed=20 + k))
+        e.fill_ghosts()
+        e.time_init(t_final=1.0, dt0=1e-4, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10)
+        e.run(steps_per_graph=3, max_accepted=3)
+        assert e.status(0).status == abi.STATUS_DONE
+        done = [e.get_state(k) for k in range(2)]
+        for _ in range(3):
+            e.step(1e-4)
+            e.accept()
+        for k in range(2):
+            assert np.array_equal(e.get_state(k), done[k])
