@@ -156,6 +156,29 @@ int pcw_sweep(pcw_handle* h, int32_t ixy, const double* dt, double* smax);
 int pcw_source(pcw_handle* h, const double* dt);
 int pcw_fill_ghosts(pcw_handle* h);
 
+/* One attempt of one domain on device frames the caller owns, allocating
+ * nothing.  It has no counterpart in the reference: it is pcw_step for
+ * callers that cannot allocate device memory (code under stream capture,
+ * serving loops, harnesses that forbid hipMalloc).
+ *   state_in, state_out: device frames (256-byte aligned, not overlapping) of
+ *     three planes h, hu, hv, each ny+4 rows of `pitch` elements of the
+ *     handle's layout: FP64 rows hold column i (-1 .. nx+2) at element i+1,
+ *     pitch = nx+4 rounded up to 64; FP32 rows are 64-element slots (slot 0
+ *     holds columns -1, 0 in its last two elements, slot s+1 columns
+ *     60s+1 .. 60s+60 then 4 unused elements, the last slot also nx+1, nx+2
+ *     after column nx), pitch = 64 * (ceil(nx/60) + 1).  state_out receives
+ *     the new interior and physical ghost cells (not the ghost corners).
+ *   dt: the attempt's step; smax[0], smax[1]: the xi / eta sweep maxima, as
+ *     pcw_step returns them.  stream: a hipStream_t, NULL for the default.
+ *   Limits: the fused engine, source kinds NONE or FRICTION, and at most
+ *     130,560 tiles (ceil(nx/60) * ceil(ny/tile_rows); tile_rows 0 means 64
+ *     rows in FP32, 128 in FP64).
+ * Descriptor, controller record and tile list live in a 2 MiB device array
+ * per GPU that each call rewrites; calls that overlap on one GPU race.
+ * Returns as pcw_step; pcw_last_error(NULL) describes a failure. */
+int pcw_step_noalloc(const pcw_params* p, const pcw_domain* d, const void* state_in,
+                     void* state_out, double dt, double* smax, void* stream);
+
 /* ---- device-controlled run (batched / persistent path) ------------------
  * Initialises the device replica of time_control for every instance. */
 int pcw_time_init(pcw_handle* h, const pcw_time* t);

@@ -91,6 +91,8 @@ SIGNATURES = {
     "pcw_last_elapsed_ms": ([_h, _dp], C.c_int),
     "pcw_kernel_stats": ([_h, _dp, _P(C.c_int64)], C.c_int),
     "pcw_autotune": ([_h, C.c_int32, _P(C.c_int32), _P(C.c_int32)], C.c_int),
+    "pcw_step_noalloc": ([_P(PcwParams), _P(PcwDomain), C.c_void_p, C.c_void_p, C.c_double, _dp,
+                          C.c_void_p], C.c_int),
 }
 
 _lib = None
@@ -124,6 +126,64 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
 
+def frame_layout(precision, nx, ny):
+    """Device layout of one state frame (pcw_step_noalloc's state_in/out):
+    (pitch, element positions of the frame columns -1..nx+2 within a row,
+    numpy dtype).  Each of the planes h, hu, hv is ny+4 rows of pitch."""
+    if precision == FP64:
+        pitch = (nx + 4 + 63) // 64 * 64
+        return pitch, np.arange(nx + 4), np.float64
+    nslot = (nx + 59) // 60
+    i = np.arange(-1, nx + 3)
+    s = np.minimum(np.maximum(i - 1, 0) // 60, nslot - 1)
+    pos = np.where(i <= 0, 63 + i, 64 * (s + 1) + (i - 1 - 60 * s))
+    return 64 * (nslot + 1), pos, np.float32
+
+
+def frame_to_planes(q, precision):
+    """Frame (ny+4, nx+4, 3) -> contiguous planes (3, ny+4, pitch) of the
+    device layout; unused elements are 0."""
+    ny, nx = q.shape[0] - 4, q.shape[1] - 4
+    pitch, pos, dt = frame_layout(precision, nx, ny)
+    out = np.zeros((3, ny + 4, pitch), dtype=dt)
+    for k in range(3):
+        out[k][:, pos] = q[..., k]
+    return out
+
+
+def planes_to_frame(planes, precision, nx, ny):
+    """Inverse of frame_to_planes, as doubles."""
+    pitch, pos, _ = frame_layout(precision, nx, ny)
+    p = np.asarray(planes).reshape(3, ny + 4, pitch)
+    q = np.zeros((ny + 4, nx + 4, 3))
+    for k in range(3):
+        q[..., k] = p[k][:, pos]
+    return q
+
+
+def step_noalloc(params: "PcwParams", dom: "Domain", state_in: int, state_out: int, dt: float,
+                 stream: int | None = None):
+    """pcw_step_noalloc on device addresses state_in/state_out (ints);
+    returns (rc, smax[2], message)."""
+    L = load()
+    smax = np.zeros(2)
+    d = dom.c()
+    rc = L.pcw_step_noalloc(C.byref(params), C.byref(d), C.c_void_p(state_in),
+                            C.c_void_p(state_out), dt, _ptr(smax), C.c_void_p(stream))
+    msg = L.pcw_last_error(None).decode() if rc else ""
+    return rc, smax, msg
+
+
+def make_params(precision=FP64, engine=FUSED, higher_order=True, use_limiter=True, limiter=4,
+                grav=GRAV_REF, eps=1e-10, dry_tol=0.0, tile_rows=0, waves_per_block=0):
+    p = PcwParams()
+    p.precision, p.engine = precision, engine
+    p.higher_order, p.use_limiter, p.limiter = int(higher_order), int(use_limiter), limiter
+    p.tile_rows, p.waves_per_block = tile_rows, waves_per_block
+    p.grav, p.eps, p.dry_tol = grav, eps, dry_tol
+    return p
+
+
 class Domain:
     """One instance's settings (control.in's mesh and BC lines)."""
 
@@ -155,11 +215,8 @@ class Engine:
                  tile_rows=0, waves_per_block=0):
         self.L = load()
         self.domains = list(domains)
-        p = PcwParams()
-        p.precision, p.engine = precision, engine
-        p.higher_order, p.use_limiter, p.limiter = int(higher_order), int(use_limiter), limiter
-        p.tile_rows, p.waves_per_block = tile_rows, waves_per_block
-        p.grav, p.eps, p.dry_tol = grav, eps, dry_tol
+        p = make_params(precision, engine, higher_order, use_limiter, limiter, grav, eps, dry_tol,
+                        tile_rows, waves_per_block)
         arr = (PcwDomain * len(self.domains))(*[d.c() for d in self.domains])
         h = C.c_void_p()
         rc = self.L.pcw_create(C.byref(p), arr, len(self.domains), C.byref(h))

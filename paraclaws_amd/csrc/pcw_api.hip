@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -57,6 +58,89 @@ struct pcw_handle {
   long long launches = 0;
   int max_nx = 0, max_ny = 0, max_side = 0;
 };
+
+// message of the last failed entry point that has no handle (pcw_step_noalloc)
+thread_local char g_loose_error[256] = {0};
+
+namespace pcw {
+
+void set_loose_error(const char* msg) {
+  std::snprintf(g_loose_error, sizeof g_loose_error, "%s", msg);
+}
+
+// True when the identity map's metrics, rounded as the FP64 kernel computes
+// them (kappa_exact, gamma_xi_exact, gamma_eta_exact), are exactly 1 on every
+// frame cell: the FP64 kernel then skips them (kMetUnit), with the same bits.
+// kappa depends on the column only through the rounded cell width x2 - x1
+// (its other two x-terms are x2 - x2 and x1 - x1, i.e. zero), so it is
+// evaluated on one column per distinct width.  Allocates nothing.
+bool metrics_are_unit(const KInst& I) {
+  const Mesh M{I.xi_min, I.eta_min, I.dxi, I.deta};
+  constexpr int kMaxWidths = 16;
+  double widths[kMaxWidths];
+  int cols[kMaxWidths];
+  int nw = 0;
+  for (int i = -1; i <= I.nx + 2; ++i) {
+    if (gamma_eta_exact(M, i) != 1.0) return false;
+    const double x1 = corner_x1(M, i), w = (x1 + M.dxi) - x1;
+    bool seen = false;
+    for (int k = 0; k < nw; ++k) seen = seen || widths[k] == w;
+    if (!seen) {
+      // a uniform grid rounds to one or a few widths; many widths mean metrics
+      // that are not all 1 anyway (gamma_eta = width / dxi), so stop early
+      if (nw == kMaxWidths) return false;
+      widths[nw] = w;
+      cols[nw++] = i;
+    }
+  }
+  for (int j = -1; j <= I.ny + 2; ++j) {
+    if (gamma_xi_exact(M, j) != 1.0) return false;
+    for (int k = 0; k < nw; ++k)
+      if (kappa_exact(M, cols[k], j) != 1.0) return false;
+  }
+  return true;
+}
+
+// Descriptor of one instance from its settings: frame layout (FP32 slot rows,
+// FP64 plain rows), spacings, BCs, extrapolation ratios and metric kind.
+// Offsets of the two buffers, bathymetry, metrics and tiles are the caller's.
+// Allocates nothing.
+void describe_instance(const pcw_params& p, const pcw_domain& d, KInst& I) {
+  std::memset(&I, 0, sizeof I);
+  I.nx = d.nx;
+  I.ny = d.ny;
+  if (p.precision == PCW_FP32) {
+    I.slot_rows = 1;
+    I.nslot = (d.nx + kStripOut - 1) / kStripOut;
+    I.pitch = 64 * (I.nslot + 1);
+  } else {
+    I.pitch = (int)(((long long)d.nx + 4 + 63) / 64 * 64);
+  }
+  I.plane = (long long)I.pitch * (d.ny + 4);
+  for (int s = 0; s < 4; ++s) I.bc[s] = d.bc[s];
+  I.source = d.source;
+  I.dxi = (d.xi_max - d.xi_min) / d.nx;   // SMOD_mesh_manipulation.f90:28-29
+  I.deta = (d.eta_max - d.eta_min) / d.ny;
+  I.xi_min = d.xi_min;
+  I.eta_min = d.eta_min;
+  I.manning = d.manning_n;
+  I.unit_met = p.precision == PCW_FP64 && metrics_are_unit(I);
+  auto c = [](double lo, double dd, int i) { return lo + (double)(i - 1) * dd + dd / 2; };
+  for (int kk = 1; kk <= 2; ++kk) {
+    const double xl = c(I.xi_min, I.dxi, 1), xl2 = c(I.xi_min, I.dxi, 2);
+    const double xr = c(I.xi_min, I.dxi, d.nx), xr2 = c(I.xi_min, I.dxi, d.nx - 1);
+    const double yb = c(I.eta_min, I.deta, 1), yb2 = c(I.eta_min, I.deta, 2);
+    const double yt = c(I.eta_min, I.deta, d.ny), yt2 = c(I.eta_min, I.deta, d.ny - 1);
+    I.gr[kk - 1] = (float)((c(I.xi_min, I.dxi, 1 - kk) - xl) / (xl2 - xl));
+    I.gr[1 + kk] = (float)((c(I.xi_min, I.dxi, d.nx + kk) - xr) / (xr2 - xr));
+    I.gr[3 + kk] = (float)((c(I.eta_min, I.deta, 1 - kk) - yb) / (yb2 - yb));
+    I.gr[5 + kk] = (float)((c(I.eta_min, I.deta, d.ny + kk) - yt) / (yt2 - yt));
+  }
+  I.boff = -1;
+  I.moff = -1;
+}
+
+}  // namespace pcw
 
 namespace {
 
@@ -357,35 +441,6 @@ int step_bad_code(pcw_handle* h) {
   return PCW_OK;
 }
 
-// True when the identity map's metrics, rounded as the FP64 kernel computes
-// them (kappa_exact, gamma_xi_exact, gamma_eta_exact), are exactly 1 on every
-// frame cell: the FP64 kernel then skips them (kMetUnit), with the same bits.
-// kappa depends on the column only through the rounded cell width x2 - x1
-// (its other two x-terms are x2 - x2 and x1 - x1, i.e. zero), so it is
-// evaluated on one column per distinct width.
-bool metrics_are_unit(const KInst& I) {
-  const Mesh M{I.xi_min, I.eta_min, I.dxi, I.deta};
-  std::vector<double> widths;
-  std::vector<int> cols;
-  for (int i = -1; i <= I.nx + 2; ++i) {
-    if (gamma_eta_exact(M, i) != 1.0) return false;
-    const double x1 = corner_x1(M, i), w = (x1 + M.dxi) - x1;
-    if (std::find(widths.begin(), widths.end(), w) == widths.end()) {
-      // a uniform grid rounds to one or a few widths; many widths mean metrics
-      // that are not all 1 anyway (gamma_eta = width / dxi), so stop early
-      if (widths.size() == 16) return false;
-      widths.push_back(w);
-      cols.push_back(i);
-    }
-  }
-  for (int j = -1; j <= I.ny + 2; ++j) {
-    if (gamma_xi_exact(M, j) != 1.0) return false;
-    for (int i : cols)
-      if (kappa_exact(M, i, j) != 1.0) return false;
-  }
-  return true;
-}
-
 // The instantiated graph of `len` device-controlled attempts, captured on
 // first use.  At most a few lengths are kept (a run uses two).
 int graph_of(pcw_handle* h, int len, hipGraphExec_t* out) {
@@ -423,7 +478,10 @@ int graph_of(pcw_handle* h, int len, hipGraphExec_t* out) {
 
 extern "C" {
 
-const char* pcw_last_error(const pcw_handle* h) { return h ? h->err.c_str() : "null handle"; }
+const char* pcw_last_error(const pcw_handle* h) {
+  if (h) return h->err.c_str();
+  return g_loose_error[0] ? g_loose_error : "null handle";
+}
 
 int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains,
                pcw_handle** out) {
@@ -469,49 +527,15 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
         return PCW_ERR_BC;
       }
     KInst& I = h->hinst[k];
-    std::memset(&I, 0, sizeof I);
-    I.nx = d.nx;
-    I.ny = d.ny;
-    if (p->precision == PCW_FP32) {
-      I.slot_rows = 1;
-      I.nslot = (d.nx + kStripOut - 1) / kStripOut;
-      I.pitch = 64 * (I.nslot + 1);
-    } else {
-      I.pitch = (int)round_up(d.nx + 4, 64);
-    }
-    I.plane = (long long)I.pitch * (d.ny + 4);
+    describe_instance(*p, d, I);
     I.off[0] = base;
     I.off[1] = base + 3 * I.plane;
     base += 6 * I.plane;
-    for (int s = 0; s < 4; ++s) I.bc[s] = d.bc[s];
-    I.source = d.source;
-    I.dxi = (d.xi_max - d.xi_min) / d.nx;   // SMOD_mesh_manipulation.f90:28-29
-    I.deta = (d.eta_max - d.eta_min) / d.ny;
-    I.xi_min = d.xi_min;
-    I.eta_min = d.eta_min;
-    I.manning = d.manning_n;
-    I.unit_met = p->precision == PCW_FP64 && metrics_are_unit(I);
-    {
-      auto c = [](double lo, double dd, int i) { return lo + (double)(i - 1) * dd + dd / 2; };
-      for (int kk = 1; kk <= 2; ++kk) {
-        const double xl = c(I.xi_min, I.dxi, 1), xl2 = c(I.xi_min, I.dxi, 2);
-        const double xr = c(I.xi_min, I.dxi, d.nx), xr2 = c(I.xi_min, I.dxi, d.nx - 1);
-        const double yb = c(I.eta_min, I.deta, 1), yb2 = c(I.eta_min, I.deta, 2);
-        const double yt = c(I.eta_min, I.deta, d.ny), yt2 = c(I.eta_min, I.deta, d.ny - 1);
-        I.gr[kk - 1] = (float)((c(I.xi_min, I.dxi, 1 - kk) - xl) / (xl2 - xl));
-        I.gr[1 + kk] = (float)((c(I.xi_min, I.dxi, d.nx + kk) - xr) / (xr2 - xr));
-        I.gr[3 + kk] = (float)((c(I.eta_min, I.deta, 1 - kk) - yb) / (yb2 - yb));
-        I.gr[5 + kk] = (float)((c(I.eta_min, I.deta, d.ny + kk) - yt) / (yt2 - yt));
-      }
-    }
     if (d.source == PCW_SRC_BATHY || d.source == PCW_SRC_BATHY_FRICTION) {
       I.boff = bbase;
       bbase += I.plane;
       any_bathy = true;
-    } else {
-      I.boff = -1;
     }
-    I.moff = -1;
     h->max_nx = std::max(h->max_nx, d.nx);
     h->max_ny = std::max(h->max_ny, d.ny);
     h->max_side = std::max(h->max_side, d.nx + d.ny);

@@ -85,4 +85,10 @@ hipError_t launch_totals(const T* pool, const KInst* dinst, const InstState* dst
 hipError_t launch_set_dt(InstState* dst, const double* ddt, int n, int controlled, hipStream_t s);
 hipError_t launch_flip(InstState* dst, int n, hipStream_t s);
 
+// Instance descriptors on the host (pcw_api.hip); neither allocates.
+bool metrics_are_unit(const KInst& I);
+void describe_instance(const pcw_params& p, const pcw_domain& d, KInst& I);
+// error message of the last failed call without a handle, on this thread
+void set_loose_error(const char* msg);
+
 }  // namespace pcw

@@ -9,6 +9,7 @@ import re
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -88,3 +89,39 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
     assert r.stdout == ""
+
+
+def test_step_noalloc_rejects_bad_arguments_before_touching_the_gpu():
+    """pcw_step_noalloc checks its arguments before any HIP call (the device
+    addresses below are never dereferenced) and reports through
+    pcw_last_error(NULL)."""
+    from paraclaws_amd import abi
+
+    p = abi.make_params(abi.FP32)
+    rc, _, msg = abi.step_noalloc(p, abi.Domain(64, 64, source=abi.SRC_BATHY), 0x10000, 0x900000, 1e-3)
+    assert rc == abi.PCW_ERR_ARG and "bathymetry" in msg
+    rc, _, msg = abi.step_noalloc(p, abi.Domain(64, 64), 0x10010, 0x900000, 1e-3)
+    assert rc == abi.PCW_ERR_ARG and "aligned" in msg
+    rc, _, msg = abi.step_noalloc(p, abi.Domain(64, 64), 0x10000, 0x10100, 1e-3)
+    assert rc == abi.PCW_ERR_ARG and "overlap" in msg
+    rc, _, msg = abi.step_noalloc(p, abi.Domain(64, 64, bc=(1, 2, 2, 2)), 0x10000, 0x900000, 1e-3)
+    assert rc == abi.PCW_ERR_BC
+    rc, _, msg = abi.step_noalloc(abi.make_params(abi.FP64, tile_rows=2), abi.Domain(60 * 600, 1000),
+                                  0x10000000, 0x80000000000, 1e-3)
+    assert rc == abi.PCW_ERR_ARG and "tiles" in msg
+    rc, _, msg = abi.step_noalloc(abi.make_params(abi.FP64, engine=abi.STAGED), abi.Domain(64, 64),
+                                  0x10000, 0x900000, 1e-3)
+    assert rc == abi.PCW_ERR_ARG
+
+
+def test_frame_layout_round_trip():
+    from paraclaws_amd import abi
+
+    rng = np.random.default_rng(0)
+    for prec in (abi.FP64, abi.FP32):
+        for nx, ny in ((2, 3), (60, 5), (61, 4), (121, 7)):
+            q = rng.normal(size=(ny + 4, nx + 4, 3)).astype(np.float32).astype(np.float64)
+            planes = abi.frame_to_planes(q, prec)
+            pitch, pos, _ = abi.frame_layout(prec, nx, ny)
+            assert len(set(pos.tolist())) == nx + 4 and pos.max() < pitch
+            assert np.array_equal(abi.planes_to_frame(planes, prec, nx, ny), q)
