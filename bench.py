@@ -344,8 +344,16 @@ def main():
     achieved = BYTES_PER_CELL_UPDATE_F32 * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
     traffic, traffic_tiles = load_traffic(n)
     e32.close()
+    # what HBM sustains for the step's compulsory traffic: three planes of the
+    # frame streamed in and three out, no stencil (pcw_stream_bandwidth)
+    plane_bytes = 4 * 64 * ((n + 59) // 60 + 1) * (n + 4)
+    att_best, att_med = abi.stream_bandwidth(plane_bytes, reps=10)
+    log(f"  attainable (3-plane stream, {6 * plane_bytes / 1e9:.2f} GB per launch): "
+        f"median {att_med:.0f} GB/s, best {att_best:.0f} GB/s; the step reaches "
+        f"{achieved / att_med:.3f} of it")
     if args.profile_only:
         print(json.dumps({"value": value, "kernel_avg_ms": kernel_ms, "achieved_gbs": achieved,
+                          "attainable_gbs": att_med,
                           "tuned": r32["tuned"], "steps": r32["accepted"]}), flush=True)
         return
 
@@ -461,7 +469,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_tile_rows_waves": traffic_tiles,
                      "kernel": KERNEL_F32, "kernel_avg_ms": kernel_ms,
-                     "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_F32},
+                     "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_F32,
+                     "attainable": att_med, "attainable_best": att_best,
+                     "attainable_kernel": "pcw_stream_bandwidth: 3 planes in, 3 out, 16-byte "
+                                          "non-temporal accesses, median of 10 launches",
+                     "frac_of_attainable": achieved / att_med},
         "cpu_baseline": cpu,
     }
     line.update(out_extra)

@@ -208,6 +208,13 @@ int pcw_last_elapsed_ms(pcw_handle* h, double* ms);
  * the graphs whose every launch made an attempt (else over all attempts).
  * launches: step launches of the run. */
 int pcw_kernel_stats(pcw_handle* h, double* avg_step_ms, int64_t* launches);
+/* Attainable bandwidth of the step's compulsory traffic on the current GPU:
+ * a stream of three planes of plane_bytes in and three out (16-byte
+ * non-temporal loads and stores, no stencil), timed with HIP events over
+ * reps launches after two warm-up launches; GB/s of the fastest and the
+ * median launch (bytes moved = 6 * plane_bytes).  Allocates and frees its
+ * six planes. */
+int pcw_stream_bandwidth(int64_t plane_bytes, int32_t reps, double* gbs_best, double* gbs_median);
 /* Autotuner: times candidate (tile_rows, waves_per_block) on the handle's
  * current state and keeps the fastest; writes the choice. */
 int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows,

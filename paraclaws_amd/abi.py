@@ -91,6 +91,7 @@ SIGNATURES = {
     "pcw_last_elapsed_ms": ([_h, _dp], C.c_int),
     "pcw_kernel_stats": ([_h, _dp, _P(C.c_int64)], C.c_int),
     "pcw_autotune": ([_h, C.c_int32, _P(C.c_int32), _P(C.c_int32)], C.c_int),
+    "pcw_stream_bandwidth": ([C.c_int64, C.c_int32, _dp, _dp], C.c_int),
     "pcw_step_noalloc": ([_P(PcwParams), _P(PcwDomain), C.c_void_p, C.c_void_p, C.c_double, _dp,
                           C.c_void_p], C.c_int),
 }
@@ -172,6 +173,15 @@ def step_noalloc(params: "PcwParams", dom: "Domain", state_in: int, state_out: i
                             C.c_void_p(state_out), dt, _ptr(smax), C.c_void_p(stream))
     msg = L.pcw_last_error(None).decode() if rc else ""
     return rc, smax, msg
+
+
+def stream_bandwidth(plane_bytes: int, reps: int = 10):
+    """pcw_stream_bandwidth: (best, median) GB/s of a 3-in/3-out plane stream."""
+    best, med = C.c_double(), C.c_double()
+    rc = load().pcw_stream_bandwidth(plane_bytes, reps, C.byref(best), C.byref(med))
+    if rc != PCW_OK:
+        raise PcwError(rc, "pcw_stream_bandwidth failed")
+    return best.value, med.value
 
 
 def make_params(precision=FP64, engine=FUSED, higher_order=True, use_limiter=True, limiter=4,

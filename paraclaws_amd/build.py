@@ -14,7 +14,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpcw_hip.so")
-SOURCES = ["pcw_fused.hip", "pcw_fused_f32.hip", "pcw_staged.hip", "pcw_api.hip", "pcw_noalloc.hip"]
+SOURCES = ["pcw_fused.hip", "pcw_fused_f32.hip", "pcw_staged.hip", "pcw_api.hip", "pcw_noalloc.hip",
+           "pcw_stream.hip"]
 ARCH = os.environ.get("PCW_OFFLOAD_ARCH", "gfx950")
 
 
