@@ -99,7 +99,7 @@ def lib():
         L.orc_tc_reset_cfl.argtypes = [P(OrcTc)]
         L.orc_tc_reset_cfl.restype = None
         L.orc_run.argtypes = [P(OrcCfg), P(OrcTc), dp, dp, C.c_int32, C.c_int32,
-                              P(OrcStepRec), C.c_int32]
+                              P(OrcStepRec), C.c_int32, C.c_int32]
         L.orc_cell_phys.argtypes = [P(OrcCfg), C.c_int, dp, dp, dp]
         L.orc_hll.argtypes = [P(OrcCfg), C.c_int, dp, dp, dp]
         L.orc_phi.argtypes = [C.c_int, C.c_double]
@@ -232,13 +232,16 @@ def source(cfg: Config, q: np.ndarray, bathy, dt: float) -> None:
 
 def run(cfg: Config, ts: TimeSettings, q: np.ndarray, bathy=None, max_accepted=0,
         max_attempts=0, tc: OrcTc | None = None):
-    """Main loop; returns (records, tc).  q is advanced in place."""
+    """Main loop; returns (records, tc).  q is advanced in place.  A run given
+    the controller `tc` of an earlier one continues it: the ghost fill that
+    precedes a run's loop is skipped, so that changes made to q between the
+    two (hooks) stay as the reference's would."""
     c = cfg.c()
     t = ts.tc() if tc is None else tc
     cap = max(max_attempts, max_accepted * 4, 1) if (max_attempts or max_accepted) else 200000
     recs = (OrcStepRec * cap)()
     n = lib().orc_run(C.byref(c), C.byref(t), _dp(q), _dp(bathy), max_accepted, max_attempts,
-                      recs, cap)
+                      recs, cap, 1 if tc is None else 0)
     if n < 0:
         raise RuntimeError(f"orc_run rc={n}")
     rows = [(r.accepted, r.snapshot, r.dt_used, r.cfl, r.time, r.dt_next) for r in recs[:min(n, cap)]]

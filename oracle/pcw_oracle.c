@@ -741,7 +741,7 @@ int orc_source(const orc_cfg* c, double* vars, const double* bathy, double dt) {
  * source, ghost fill, updateTime, snapshot index, reset_CFL. */
 int orc_run(const orc_cfg* c, orc_tc* tc, double* vars, const double* bathy,
             int32_t max_accepted, int32_t max_attempts, orc_step_rec* rec,
-            int32_t rec_cap) {
+            int32_t rec_cap, int32_t fill_first) {
 #ifdef _OPENMP
   if (c->nthreads > 0) omp_set_num_threads(c->nthreads);
 #endif
@@ -750,7 +750,9 @@ int orc_run(const orc_cfg* c, orc_tc* tc, double* vars, const double* bathy,
   double* old = malloc((size_t)nv * sizeof(double));
   work_t w;
   if (!old || work_alloc(&w, &f)) { free(old); return -1000; }
-  int rc = orc_fill_ghosts(c, vars);
+  /* the ghost fill before the loop (src/main.f90:161, :189); a run that
+   * continues an earlier one (fill_first = 0) starts from the state as it is */
+  int rc = fill_first ? orc_fill_ghosts(c, vars) : 0;
   int attempts = 0, accepted = 0;
   while (rc == 0 && tc->time < tc->time_final) {
     if (max_accepted > 0 && accepted >= max_accepted) break;

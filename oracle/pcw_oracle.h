@@ -122,7 +122,7 @@ void orc_tc_reset_cfl(orc_tc* tc);
  * number of attempts, or <0 on error. */
 int orc_run(const orc_cfg* c, orc_tc* tc, double* vars, const double* bathy,
             int32_t max_accepted, int32_t max_attempts, orc_step_rec* rec,
-            int32_t rec_cap);
+            int32_t rec_cap, int32_t fill_first);
 
 /* Test hooks: per-cell speed and flux of one sweep direction (returns the
  * reference's error condition), one HLL interface (out: lambda1, lambda2,

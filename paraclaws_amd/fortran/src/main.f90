@@ -4,14 +4,16 @@
 program paraCLAWs_gpu
     use MOD_decomposition, only: decomp_info, initialize_decomposition
     use MOD_user_subroutines, only: setup_user_input, set_initial_conditions, &
-        setup_auxiliary_quantities, after_step
+        setup_auxiliary_quantities, after_step, source_function, host_state_hooks
     use MOD_time_control, only: time_control
     use MOD_domain, only: update_rank_boundaries, solution_fields, mesh_fields, write_base_info, &
         setup_mesh
     use MOD_solve, only: solve_dimensional_split, integrate_source_term, execute_post
     use MOD_IO
     use MOD_work_arrays, only: work_arrays
-    use MOD_device, only: device_upload, device_download, device_commit, device_totals
+    use MOD_device, only: device_upload, device_download, device_commit, device_totals, &
+        device_source_kind
+    use MOD_error_handling, only: abort_program
     implicit none
 
     integer :: ierr, exitstat, cmdstat
@@ -23,6 +25,8 @@ program paraCLAWs_gpu
     double precision :: totals(3)
     integer :: n_accepted, n_rejected
     integer :: count0, count1, count_rate
+    integer :: i, j, ilow, ihigh, jlow, jhigh
+    logical :: state_hooks
 
 1   format('paraCLAWs - time: ',f10.8,' current dt: ',f10.8,' CFL(max)=',f10.5,' next dt: ',f10.8,' => snapshot: ', i4)
 2   format('paraCLAWs - time: ',f10.8,' current dt: ',f10.8,' CFL(max)=',f10.5,' next dt: ',f10.8)
@@ -45,6 +49,20 @@ program paraCLAWs_gpu
 
     call setup_auxiliary_quantities(decomposition, mesh, solution)
     call set_initial_conditions(decomposition, mesh, solution)
+    ! the device cannot call source_function per cell (src/solver/MOD_solve.f90:
+    ! 319-327); without a device source selected in SMOD_device_physics, a user
+    ! source_function that is nonzero on the initial state is rejected
+    if (device_source_kind == 0) then
+        call decomposition%get_local_block_bounds(ilow, ihigh, jlow, jhigh)
+        do j = jlow, jhigh
+            do i = ilow, ihigh
+                if (any(source_function(i, j, mesh, solution, decomposition) /= 0.d0)) &
+                    call abort_program("source_function returns a nonzero source, which the GPU step " // &
+                        "cannot evaluate per cell: select a built-in device source in SMOD_device_physics")
+            end do
+        end do
+    end if
+    state_hooks = host_state_hooks()
     call device_upload(solution%vars)
     call update_rank_boundaries(decomposition, mesh, solution)
     call device_download(solution%vars)
@@ -85,6 +103,7 @@ program paraCLAWs_gpu
         if (t_ctrl%time >= t_ctrl%snapshot_intervals(t_ctrl%get_snapshot_index())) then
             call device_download(solution%vars)
             call execute_post(decomposition, mesh, solution)
+            call device_upload(solution%vars)     ! user_post's changes
             call writeData(decomposition, t_ctrl%time, solution)
             call write_vtk_rank(decomposition, mesh, solution, t_ctrl%time)
             call write_pvts_rank(decomposition, t_ctrl%time)
@@ -95,7 +114,9 @@ program paraCLAWs_gpu
             write(*,2) t_ctrl%time, t_ctrl%get_previous_dt(), t_ctrl%get_CFL(), t_ctrl%get_dt()
         end if
         call t_ctrl%reset_CFL()
+        if (state_hooks) call device_download(solution%vars)
         call after_step(decomposition, t_ctrl, mesh, solution)
+        if (state_hooks) call device_upload(solution%vars)
     end do
     call system_clock(count1)
 

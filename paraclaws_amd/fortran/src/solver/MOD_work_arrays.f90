@@ -19,15 +19,31 @@ contains
     subroutine allocate_work_arrays(work, decomposition, mesh)
         use MOD_decomposition, only: decomp_info
         use MOD_domain, only: mesh_fields
-        use MOD_device, only: device_create, device_set_metrics
+        use MOD_device, only: device_create, device_set_metrics, device_set_bathymetry
+        use MOD_user_subroutines, only: device_physics, device_bathymetry
         use MOD_solver_parameters
         class(work_arrays), intent(inout) :: work
         type(decomp_info), intent(in) :: decomposition
         type(mesh_fields), intent(in) :: mesh
+        integer :: source_kind, ilow, ihigh, jlow, jhigh
+        double precision :: manning_n, dry_tol
+        double precision, allocatable :: b(:,:)
+        ! the device source that replaces source_function (SMOD_device_physics)
+        call device_physics(source_kind, manning_n, dry_tol)
         call device_create(mesh%m_xi, mesh%m_eta, &
             [mesh%bc_xi_low, mesh%bc_xi_high, mesh%bc_eta_low, mesh%bc_eta_high], &
             mesh%xi_dimensions, mesh%eta_dimensions, grav, device_precision, device_engine, &
-            add_higher_order_flux_correction, use_limiters, limiter_method)
+            add_higher_order_flux_correction, use_limiters, limiter_method, &
+            source_kind, manning_n, dry_tol)
+        if (source_kind /= 0 .or. dry_tol > 0.d0) write(*,'(a,i0,a,es12.5,a,es12.5)') &
+            " device source kind ", source_kind, "  Manning n ", manning_n, "  wet/dry threshold ", dry_tol
+        if (source_kind == 1 .or. source_kind == 3) then
+            call decomposition%get_local_block_bounds(ilow, ihigh, jlow, jhigh)
+            allocate(b(ilow-2:ihigh+2, jlow-2:jhigh+2))
+            call device_bathymetry(decomposition, mesh, b)
+            call device_set_bathymetry(b)
+            deallocate(b)
+        end if
         ! a user coordinate map other than the identity: hand the engine the
         ! capacity and length ratios computed from it (SMOD_mesh_manipulation)
         if (.not. identity_map(mesh)) then

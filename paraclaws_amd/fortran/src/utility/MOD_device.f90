@@ -8,6 +8,7 @@ module MOD_device
 
     type(c_ptr), save :: handle = c_null_ptr
     integer, save :: m_xi_ = 0, m_eta_ = 0
+    integer, save :: device_source_kind = 0   ! the built-in source the engine applies
 
 contains
 
@@ -19,10 +20,12 @@ contains
     end subroutine
 
     subroutine device_create(m_xi, m_eta, bc, xi_dimensions, eta_dimensions, grav_, &
-            precision_, engine_, higher_order, use_limiter, limiter)
+            precision_, engine_, higher_order, use_limiter, limiter, source_kind, manning_n, dry_tol)
+        !! source_kind, manning_n, dry_tol: the built-in device source of
+        !! SMOD_device_physics (PCW_SRC_NONE, 0, 0: the reference's phi = 0)
         use MOD_constants, only: ud_precision
-        integer, intent(in) :: m_xi, m_eta, bc(4), precision_, engine_, limiter
-        double precision, intent(in) :: xi_dimensions(2), eta_dimensions(2), grav_
+        integer, intent(in) :: m_xi, m_eta, bc(4), precision_, engine_, limiter, source_kind
+        double precision, intent(in) :: xi_dimensions(2), eta_dimensions(2), grav_, manning_n, dry_tol
         logical, intent(in) :: higher_order, use_limiter
         type(pcw_params) :: p
         type(pcw_domain) :: d(1)
@@ -37,21 +40,22 @@ contains
         p%pad_ = 0
         p%grav = grav_
         p%eps = ud_precision
-        p%dry_tol = 0.d0
+        p%dry_tol = dry_tol
         d(1)%nx = m_xi
         d(1)%ny = m_eta
         d(1)%bc = bc
-        d(1)%source = 0
+        d(1)%source = source_kind
         d(1)%pad_ = 0
         d(1)%xi_min = xi_dimensions(1)
         d(1)%xi_max = xi_dimensions(2)
         d(1)%eta_min = eta_dimensions(1)
         d(1)%eta_max = eta_dimensions(2)
-        d(1)%manning_n = 0.d0
+        d(1)%manning_n = manning_n
         rc = pcw_create(p, d, 1_c_int32_t, handle)
         call device_check(rc, "pcw_create")
         m_xi_ = m_xi
         m_eta_ = m_eta
+        device_source_kind = source_kind
     end subroutine
 
     subroutine device_upload(vars)
@@ -69,6 +73,12 @@ contains
         double precision, intent(in), contiguous :: kappa(:,:), gamma_xi(:,:), gamma_eta(:,:)
         call device_check(pcw_set_metrics(handle, 0_c_int32_t, kappa, gamma_xi, gamma_eta), &
             "pcw_set_metrics")
+    end subroutine
+
+    subroutine device_set_bathymetry(b)
+        !! bottom elevation b(ilow-2:ihigh+2, jlow-2:jhigh+2) of a bathymetry source
+        double precision, intent(in), contiguous :: b(:,:)
+        call device_check(pcw_set_bathymetry(handle, 0_c_int32_t, b), "pcw_set_bathymetry")
     end subroutine
 
     subroutine device_fill_ghosts()

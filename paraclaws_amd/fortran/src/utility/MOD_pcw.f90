@@ -10,6 +10,8 @@ module MOD_pcw
     integer(c_int), parameter :: PCW_OK = 0, PCW_REJECTED = 1, PCW_ERR_STATE_ETA = -6
     integer(c_int), parameter :: PCW_FP64 = 0, PCW_FP32 = 1
     integer(c_int), parameter :: PCW_ENGINE_FUSED = 0, PCW_ENGINE_STAGED = 1
+    integer(c_int), parameter :: PCW_SRC_NONE = 0, PCW_SRC_BATHY = 1, PCW_SRC_FRICTION = 2, &
+        PCW_SRC_BATHY_FRICTION = 3
 
     type, bind(C) :: pcw_domain
         integer(c_int32_t) :: nx, ny
@@ -54,6 +56,14 @@ module MOD_pcw
             type(c_ptr), value :: h
             integer(c_int32_t), value :: inst
             real(c_double), intent(in) :: vars(*)
+            integer(c_int) :: rc
+        end function
+
+        function pcw_set_bathymetry(h, inst, b) bind(C, name="pcw_set_bathymetry") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: inst
+            real(c_double), intent(in) :: b(*)
             integer(c_int) :: rc
         end function
 

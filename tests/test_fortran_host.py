@@ -214,3 +214,122 @@ def test_fortran_host_stretched_grid_matches_oracle(oracle, tmp_path):
 
     m = sides_mask(n, n)
     assert np.array_equal(got[m], q[m])
+
+
+BATHY_EXAMPLE = os.path.join(FDIR, "examples", "bathymetry")
+HOOKS_CASE = os.path.join(ROOT, "tests", "fortran_hooks")
+SOURCE_CASE = os.path.join(ROOT, "tests", "fortran_source")
+
+
+def _exe(user_dir):
+    exe = os.path.join(user_dir, "build", "paraclaws_gpu")
+    if not os.path.exists(exe):
+        pytest.skip(f"Fortran host of {user_dir} not built (run __graft_entry__.build())")
+    return exe
+
+
+def test_device_physics_override_replaces_the_base_submodules():
+    """The GPU path's settings are a hook submodule like the reference's: a
+    USER_DIR file declaring SMOD_device_physics (or SMOD_setup, SMOD_user_post)
+    takes the base file's place in the compile order."""
+    base = base_sources()
+    for user_dir, names in ((HOOKS_CASE, ["SMOD_setup", "SMOD_user_post", "SMOD_device_physics",
+                                          "SMOD_set_initial_conditions"]),
+                            (BATHY_EXAMPLE, ["SMOD_device_physics", "SMOD_set_initial_conditions"])):
+        r = source_list(user_dir, base)
+        assert r.returncode == 0, r.stderr
+        got = r.stdout.split()
+        assert len(got) == len(base)
+        for i, (b, g) in enumerate(zip(base, got)):
+            stem = os.path.basename(b)[:-4]
+            if stem in names:
+                assert g == os.path.join(user_dir, stem + ".f90"), (i, g)
+            else:
+                assert g == b
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", [0, 1])
+def test_fortran_host_bathymetry_example_matches_oracle(oracle, tmp_path, engine):
+    """Config 3's physics selected through the host: the example's
+    SMOD_device_physics override (bathymetry slope, wet/dry cut-off) and
+    initial state (still water over the seeded Gaussian bathymetry, dam-break
+    step on x < 0, dry cells), outflow sides, run by the Fortran program on
+    the GPU in FP64, equal the oracle bit for bit."""
+    from paraclaws_amd import ic
+
+    exe = _exe(BATHY_EXAMPLE)
+    nx = ny = 96
+    (tmp_path / "control.in").write_text(_control_in(nx, 0.03, bc=(2, 2, 2, 2)))
+    (tmp_path / "custom.in").write_text(f"0\n{engine}\n")
+    b = ic.gaussian_bathymetry(nx, ny, seed=3)
+    b.tofile(tmp_path / "bathymetry.bin")
+    print(f"fortran host bathymetry example engine={engine}", flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "device source kind 1" in r.stdout
+    got = np.fromfile(tmp_path / "output" / "final_state.bin", dtype=np.float64).reshape(ny + 4, nx + 4, 3)
+    cfg = oracle.Config(nx=nx, ny=ny, limiter=4, bc=(2, 2, 2, 2), source=1, dry_tol=1e-6)
+    ts = oracle.TimeSettings(t_final=0.03, cfl_target=0.9, dt_max=1e-2, n_snap=10)
+    q = ic.still_water_with_step(b, nx, dry_tol=1e-6)
+    assert (q[..., 0] == 0.0).any()  # the start has dry cells
+    recs, tc = oracle.run(cfg, ts, q, bathy=np.ascontiguousarray(b))
+    from tests.helpers import sides_mask
+
+    m = sides_mask(nx, ny)
+    assert tc.time >= 0.03 - 1e-15
+    assert np.array_equal(got[m].view(np.uint64), q[m].view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_fortran_host_hooks_change_the_device_state(oracle, tmp_path):
+    """user_post (through execute_post at start-up and at every snapshot) and
+    after_step work on the current state and their changes reach the device:
+    the test case's user_post raises h by 1e-3 in every interior cell, its
+    after_step zeroes hu after the fifth accepted step.  The oracle replays
+    the same run step by step with the same changes between steps; the final
+    states are equal bit for bit (FP64)."""
+    exe = _exe(HOOKS_CASE)
+    n, t_final = 64, 0.05
+    (tmp_path / "control.in").write_text(_control_in(n, t_final))
+    (tmp_path / "custom.in").write_text("0\n0\n")
+    print("fortran host hooks", flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    got = np.fromfile(tmp_path / "output" / "final_state.bin", dtype=np.float64).reshape(n + 4, n + 4, 3)
+    cfg = oracle.Config(nx=n, ny=n, limiter=4, bc=(4, 4, 4, 4))
+    ts = oracle.TimeSettings(t_final=t_final, cfl_target=0.9, dt_max=1e-2, n_snap=10)
+    q = oracle.dam_break(cfg)
+    oracle.fill_ghosts(cfg, q)         # start-up: update_rank_boundaries
+    q[2:-2, 2:-2, 0] += 1e-3           # execute_post at t = 0
+    oracle.fill_ghosts(cfg, q)         # the boundary update before the loop
+    tc, accepted, posts = None, 0, 1
+    while tc is None or tc.time < tc.time_final:
+        recs, tc = oracle.run(cfg, ts, q, max_accepted=1, tc=tc if tc is not None else ts.tc())
+        assert recs and recs[-1][0] == 1
+        accepted += 1
+        if recs[-1][1] > 0:            # a snapshot: execute_post
+            q[2:-2, 2:-2, 0] += 1e-3
+            posts += 1
+        if accepted == 5:              # after_step
+            q[..., 1] = 0.0
+    assert posts == 11
+    from tests.helpers import sides_mask
+
+    m = sides_mask(n, n)
+    assert np.array_equal(got[m].view(np.uint64), q[m].view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_fortran_host_rejects_a_nonzero_source_function(tmp_path):
+    """A user source_function that is nonzero on the initial state, with no
+    device source selected, stops the program at start-up with a message:
+    the GPU step cannot call it per cell."""
+    exe = _exe(SOURCE_CASE)
+    (tmp_path / "control.in").write_text(_control_in(32, 0.01))
+    (tmp_path / "custom.in").write_text("0\n0\n")
+    print("fortran host nonzero source_function", flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0
+    assert "source_function returns a nonzero source" in r.stdout + r.stderr
+    assert not (tmp_path / "output" / "final_state.bin").exists()
