@@ -33,8 +33,13 @@ struct pcw_handle {
   KInst* dinst = nullptr;
   InstState* dst = nullptr;
   int4* dtiles = nullptr;
+  int dtiles_cap = 0;             // tiles dtiles holds room for
+  int* dlive = nullptr;           // device copy of ntiles, read by the step kernels
   TileSlot* dslots = nullptr;     // FP32 completion accumulators, kSlots per instance
-  int ntiles = 0;
+  int ntiles = 0;                 // tiles in the list now
+  int tile_cap = 0;               // tiles the step launches' grids are sized for
+  bool tiles_subset = false;      // the list holds the running instances only
+  int tiled_running = 0;          // running instances when the list was built
   void* pool = nullptr;
   long long pool_elems = 0;
   void* bpool = nullptr;
@@ -231,11 +236,13 @@ void row_bands(int ny, int ty, bool short_ends, bool short_all, std::vector<int2
 // (every band of the two edge strips, the first and last band of the others):
 // a launch then does not wait on its boundary tiles.
 // number of tiles build_tiles makes for tile height ty
-long long count_tiles(const pcw_handle* h, int ty) {
+long long count_tiles(const pcw_handle* h, int ty, const std::vector<char>* only = nullptr) {
   const bool shorten = h->p.precision == PCW_FP32;
   std::vector<int2> inner, edge;
   long long t = 0;
-  for (const auto& I : h->hinst) {
+  for (int k = 0; k < h->n; ++k) {
+    if (only && !(*only)[k]) continue;
+    const KInst& I = h->hinst[k];
     const int nsx = (I.nx + kStripOut - 1) / kStripOut;
     row_bands(I.ny, ty, shorten, false, inner);
     row_bands(I.ny, ty, shorten, shorten, edge);
@@ -245,13 +252,18 @@ long long count_tiles(const pcw_handle* h, int ty) {
   return t;
 }
 
-void build_tiles(pcw_handle* h, int ty) {
+// only: the instances to tile (all when null); the others get no tiles
+void build_tiles(pcw_handle* h, int ty, const std::vector<char>* only = nullptr) {
   h->htiles.clear();
   const bool shorten = h->p.precision == PCW_FP32;
   std::vector<int2> inner, edge;
   for (int k = 0; k < h->n; ++k) {
     KInst& I = h->hinst[k];
     I.tile0 = (int)h->htiles.size();
+    if (only && !(*only)[k]) {
+      I.ntiles = 0;
+      continue;
+    }
     std::vector<int> xs;
     const int nsx = (I.nx + kStripOut - 1) / kStripOut;
     for (int s = 0; s < nsx; ++s) {
@@ -276,12 +288,20 @@ void build_tiles(pcw_handle* h, int ty) {
   order_tiles(h);
 }
 
+// tile list, descriptors and live tile count to the device (between launches)
+void drop_graph(pcw_handle* h);
+
 int upload_tiles(pcw_handle* h) {
-  if (h->dtiles) HIPCHK(hipFree(h->dtiles));
-  h->dtiles = nullptr;
-  HIPCHK(hipMalloc(&h->dtiles, sizeof(int4) * std::max(1, h->ntiles)));
+  if (!h->dtiles || h->ntiles > h->dtiles_cap) {
+    drop_graph(h);  // captured launches hold the old list's address
+    if (h->dtiles) HIPCHK(hipFree(h->dtiles));
+    h->dtiles = nullptr;
+    h->dtiles_cap = std::max(1, h->ntiles);
+    HIPCHK(hipMalloc(&h->dtiles, sizeof(int4) * h->dtiles_cap));
+  }
   HIPCHK(hipMemcpy(h->dtiles, h->htiles.data(), sizeof(int4) * h->ntiles, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->dinst, h->hinst.data(), sizeof(KInst) * h->n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->dlive, &h->ntiles, sizeof(int), hipMemcpyHostToDevice));
   return PCW_OK;
 }
 
@@ -311,13 +331,13 @@ int limiter_code(const pcw_params& p) { return (p.higher_order && p.use_limiter)
 // identity-map instances whose metrics are exactly 1 in a third one).
 int enqueue_fused(pcw_handle* h) {
   const int threads = 64 * h->wpb;
-  int blocks = (h->ntiles + h->wpb - 1) / h->wpb;
+  int blocks = (h->tile_cap + h->wpb - 1) / h->wpb;
   blocks = (int)round_up(blocks, 8);
   if (h->p.precision == PCW_FP64) {
     bool has[3] = {false, false, false};
     for (const auto& I : h->hinst) has[metric_mode(I)] = true;
     FusedArgs<double> a{(double*)h->pool, (const double*)h->bpool, (const double*)h->mpool,
-                        h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
+                        h->dinst, h->dst, h->dtiles, h->dlive, h->tile_cap, h->p.higher_order,
                         h->p.dry_tol > 0.0, 0, h->p.grav, h->p.eps, h->p.dry_tol, nullptr};
     for (int mode = 0; mode < 3; ++mode) {
       if (!has[mode]) continue;
@@ -340,7 +360,7 @@ int enqueue_fused(pcw_handle* h) {
     any_nosrc = any_nosrc || (d.source == PCW_SRC_NONE && !wetdry);
   }
   FusedArgs<float> a{(float*)h->pool, (const float*)h->bpool, (const float*)h->mpool,
-                     h->dinst, h->dst, h->dtiles, h->ntiles, h->p.higher_order,
+                     h->dinst, h->dst, h->dtiles, h->dlive, h->tile_cap, h->p.higher_order,
                      wetdry, 0, h->p.grav, h->p.eps, h->p.dry_tol, h->dslots};
   for (int met = 0; met <= 1; ++met) {
     if (met ? !h->any_met : !h->any_plain) continue;
@@ -474,6 +494,61 @@ int graph_of(pcw_handle* h, int len, hipGraphExec_t* out) {
   return PCW_OK;
 }
 
+// The tile list of every instance at the tuned height, after a run left a
+// list of its running instances only.
+int ensure_full_tiles(pcw_handle* h) {
+  if (!h->tiles_subset) return PCW_OK;
+  build_tiles(h, h->tile_rows);
+  h->tiles_subset = false;
+  if (h->ntiles != h->tile_cap) {
+    h->tile_cap = h->ntiles;
+    drop_graph(h);
+  }
+  return upload_tiles(h);
+}
+
+// Between the launches of a device-controlled run of several instances: when
+// instances have finished, the tile list is rebuilt for the running ones, so
+// that no launch dispatches tiles that exit at once, and with shorter tiles
+// (down to kMinRetileRows) while the running instances' tiles would not fill
+// the GPU's resident waves once (kResidentWavesPerCU per CU).  The step
+// launches read the list's length from device memory, so a captured graph
+// stays valid; it is captured again, for a smaller grid, once the list is
+// below a quarter of the grid.  Cell values do not depend on the tiling.
+constexpr int kMinRetileRows = 16;
+constexpr int kResidentWavesPerCU = 16;
+int maybe_retile(pcw_handle* h) {
+  if (h->p.engine != PCW_ENGINE_FUSED || h->n < 2) return PCW_OK;
+  std::vector<char> run(h->n, 0);
+  int nrun = 0;
+  for (int k = 0; k < h->n; ++k) {
+    run[k] = h->hst[k].status == PCW_STATUS_RUNNING;
+    nrun += run[k];
+  }
+  if (nrun == 0 || nrun == h->tiled_running) return PCW_OK;
+  int cu = 0;
+  (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, 0);
+  const long long target = (long long)std::max(cu, 1) * kResidentWavesPerCU;
+  int ty = h->tile_rows;
+  while (ty > kMinRetileRows && count_tiles(h, ty, &run) < target)
+    ty = std::max(kMinRetileRows, ty * 3 / 4);
+  while (count_tiles(h, ty, &run) > h->tile_cap) ty += std::max(1, ty / 4);
+  build_tiles(h, ty, &run);
+  h->tiles_subset = true;
+  h->tiled_running = nrun;
+  // an instance without tiles takes no attempt, so the FP64 ghost fill after
+  // each step must not refill it (its first tile would have cleared this)
+  for (int k = 0; k < h->n; ++k)
+    if (!run[k]) h->hst[k].ghosts_due = 0;
+  int rc = push_states(h);
+  if (rc) return rc;
+  if ((long long)h->ntiles * 4 < h->tile_cap) {
+    h->tile_cap = std::max(1, h->ntiles);
+    drop_graph(h);
+  }
+  return upload_tiles(h);
+}
+
 }  // namespace
 
 extern "C" {
@@ -573,7 +648,9 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
     HIPCHK(hipMalloc(&h->work, (size_t)welems * sizeof(double)));
     HIPCHK(hipMemset(h->work, 0, (size_t)welems * sizeof(double)));
   }
+  HIPCHK(hipMalloc(&h->dlive, sizeof(int)));
   build_tiles(h, h->tile_rows);
+  h->tile_cap = h->ntiles;
   int rc = upload_tiles(h);
   if (rc) return rc;
   return push_states(h);
@@ -584,7 +661,7 @@ int pcw_destroy(pcw_handle* h) {
   drop_graph(h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->pool, h->bpool, h->mpool, h->dinst, h->dst, h->dtiles, h->dslots, h->dsnaps, h->dlog,
-                  h->work, h->dscratch};
+                  h->work, h->dscratch, h->dlive};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -703,7 +780,9 @@ int pcw_set_metrics(pcw_handle* h, int32_t inst, const double* kappa, const doub
 // ------------------------------------------------- host-controlled stepping
 int pcw_step(pcw_handle* h, const double* dt, double* smax) {
   if (!h || !dt) return fail(h, PCW_ERR_ARG, "bad argument");
-  int rc = set_dts(h, dt, 0);
+  int rc = ensure_full_tiles(h);
+  if (rc) return rc;
+  rc = set_dts(h, dt, 0);
   if (rc) return rc;
   if (h->p.engine == PCW_ENGINE_FUSED) {
     rc = enqueue_fused(h);
@@ -874,6 +953,18 @@ int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts, int64_
   }
   rc = push_states(h);
   if (rc) return rc;
+  // a list of an earlier run's running instances serves while every running
+  // instance has tiles in it (pcw_time_init or a raised max_accepted can
+  // restart others)
+  bool covered = true;
+  for (int k = 0; k < h->n; ++k)
+    covered = covered && (h->hst[k].status != PCW_STATUS_RUNNING || h->hinst[k].ntiles > 0);
+  if (!covered) {
+    rc = ensure_full_tiles(h);
+    if (rc) return rc;
+    h->tiled_running = 0;
+    for (int k = 0; k < h->n; ++k) h->tiled_running += h->hst[k].status == PCW_STATUS_RUNNING;
+  }
   long long before = 0;
   for (int k = 0; k < h->n; ++k)
     before = std::max(before, h->hst[k].accepted + h->hst[k].rejected);
@@ -918,6 +1009,8 @@ int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts, int64_
       work_ms += ms;
       work_launches += len;
     }
+    rc = maybe_retile(h);
+    if (rc) return rc;
   }
   long long after = 0;
   for (int k = 0; k < h->n; ++k)
@@ -1026,6 +1119,7 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
     const int tr = c.first, wv = c.second;
     h->wpb = wv;
     build_tiles(h, tr);
+    h->tile_cap = h->ntiles;
     rc = upload_tiles(h);
     if (rc) return rc;
     rc = enqueue_fused(h);  // warm-up
@@ -1048,6 +1142,8 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
   h->tile_rows = br;
   h->wpb = bw;
   build_tiles(h, br);
+  h->tile_cap = h->ntiles;
+  h->tiles_subset = false;
   rc = upload_tiles(h);
   if (rc) return rc;
   h->hst = saved;

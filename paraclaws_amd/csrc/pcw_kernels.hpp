@@ -39,7 +39,9 @@ struct FusedArgs {
   const KInst* inst;
   InstState* st;
   const int4* tiles;   // (instance, x0, y0, y1): output columns x0..x0+59, rows y0..y1
-  int ntiles;
+  const int* live;     // number of tiles in the list now (device memory: the host may shrink
+                       // the list between launches of a captured graph)
+  int ntiles;          // tiles the launch's grid was sized for
   int ho;              // add_higher_order_flux_correction
   int wetdry;          // dry_tol > 0
   int pad;

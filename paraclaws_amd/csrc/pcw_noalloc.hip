@@ -25,13 +25,13 @@ namespace pcw {
 namespace {
 
 // scratch layout (bytes): descriptor, controller record, kSlots completion
-// accumulators, tile list
+// accumulators, tile count, tile list
 constexpr size_t kScratchBytes = size_t(2) << 20;
-constexpr size_t kOffInst = 0, kOffState = 512, kOffSlots = 1024, kOffTiles = 8192;
+constexpr size_t kOffInst = 0, kOffState = 512, kOffSlots = 1024, kOffLive = 6144, kOffTiles = 8192;
 constexpr long long kMaxTiles = (long long)((kScratchBytes - kOffTiles) / sizeof(int4));
 static_assert(sizeof(KInst) <= kOffState - kOffInst, "descriptor does not fit");
 static_assert(sizeof(InstState) <= kOffSlots - kOffState, "controller record does not fit");
-static_assert(sizeof(TileSlot) * kSlots <= kOffTiles - kOffSlots, "accumulators do not fit");
+static_assert(sizeof(TileSlot) * kSlots <= kOffLive - kOffSlots, "accumulators do not fit");
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -67,6 +67,7 @@ __global__ void k_noalloc_setup(unsigned char* scr, KInst I, double dt, int nsx,
     S.last_ok = 1;
     S.tc.dt = dt;
     *reinterpret_cast<InstState*>(scr + kOffState) = S;
+    *reinterpret_cast<int*>(scr + kOffLive) = I.ntiles;
   }
   constexpr int kSlotWords = (int)(sizeof(TileSlot) * kSlots / sizeof(unsigned));
   if (t < kSlotWords) reinterpret_cast<unsigned*>(scr + kOffSlots)[t] = 0u;
@@ -147,18 +148,19 @@ extern "C" int pcw_step_noalloc(const pcw_params* p, const pcw_domain* d, const 
   InstState* dst = reinterpret_cast<InstState*>(scr + kOffState);
   const int4* dtiles = reinterpret_cast<const int4*>(scr + kOffTiles);
   TileSlot* dslots = reinterpret_cast<TileSlot*>(scr + kOffSlots);
+  const int* dlive = reinterpret_cast<const int*>(scr + kOffLive);
   const int wpb = (p->waves_per_block >= 1 && p->waves_per_block <= 4) ? p->waves_per_block : 2;
   const int blocks = (int)((ntiles + wpb - 1) / wpb + 7) / 8 * 8;
   const int lim = (p->higher_order && p->use_limiter) ? p->limiter : 0;
   const bool wetdry = p->dry_tol > 0.0;
   if (p->precision == PCW_FP32) {
     FusedArgs<float> a{static_cast<float*>(const_cast<void*>(state_in)), nullptr, nullptr, dinst, dst,
-                       dtiles, (int)ntiles, p->higher_order, wetdry, 0, p->grav, p->eps, p->dry_tol,
-                       dslots};
+                       dtiles, dlive, (int)ntiles, p->higher_order, wetdry, 0, p->grav, p->eps,
+                       p->dry_tol, dslots};
     e = launch_fused_f32(lim, false, d->source != PCW_SRC_NONE || wetdry, a, blocks, 64 * wpb, s);
   } else {
     FusedArgs<double> a{static_cast<double*>(const_cast<void*>(state_in)), nullptr, nullptr, dinst,
-                        dst, dtiles, (int)ntiles, p->higher_order, wetdry, 0, p->grav, p->eps,
+                        dst, dtiles, dlive, (int)ntiles, p->higher_order, wetdry, 0, p->grav, p->eps,
                         p->dry_tol, nullptr};
     e = launch_fused_f64(lim, metric_mode(I), a, blocks, 64 * wpb, s);
     if (e == hipSuccess)  // ghost cells of the new state (the pending buffer)
