@@ -149,28 +149,32 @@ def run_ensemble(abi, ic, spg):
         upd += st.accepted * n * n
         acc_max = max(acc_max, st.accepted + st.rejected)
     e.close()
-    return dict(value=upd / wall, unit="cell-updates/s", instances=len(members),
+    value = upd / wall
+    return dict(value=value, unit="cell-updates/s", instances=len(members),
                 cells=int(sum(n * n for n, _ in members)), cell_updates=int(upd), wall_s=wall,
-                max_attempts=acc_max, tuned=tuned)
+                max_attempts=acc_max, tuned=tuned,
+                roofline_frac_wall_24B=value * BYTES_PER_CELL_UPDATE_F32 / 1e9 / HBM_PEAK_GBS)
 
 
-def run_bathymetry(abi, ic, n, steps, spg):
+def run_bathymetry(abi, ic, n, steps, spg, tiles=None, warmup=10):
     """Config 3: FP32 n x n over a seeded Gaussian-sum bathymetry (seed 3),
     still water h + b = 1 with dry cells (tol 1e-6), dam-break step h += 1 on
-    x < 0, outflow (zero-gradient) BCs; FP64 mass total before and after."""
+    x < 0, outflow (zero-gradient) BCs; FP64 mass total before and after.
+    tiles: (rows, waves per block) instead of the autotuner's."""
     b = ic.gaussian_bathymetry(n, n, seed=3)
     q = ic.still_water_with_step(b, n, dry_tol=1e-6)
     dom = abi.Domain(n, n, bc=(2, 2, 2, 2), source=abi.SRC_BATHY)
-    e = abi.Engine([dom], precision=abi.FP32, limiter=4, dry_tol=1e-6)
+    e = abi.Engine([dom], precision=abi.FP32, limiter=4, dry_tol=1e-6,
+                   **(dict(tile_rows=tiles[0], waves_per_block=tiles[1]) if tiles else {}))
     e.set_state(0, q)
     e.set_bathymetry(0, b)
     e.fill_ghosts()
     del q, b
     m0 = e.totals(0)[0]
     e.time_init(**TIME_KW)
-    tuned = e.autotune(steps=3)
-    e.run(steps_per_graph=spg, max_accepted=10)
-    m_after_warm = None
+    tuned = tuple(tiles) if tiles else e.autotune(steps=3)
+    if warmup:
+        e.run(steps_per_graph=spg, max_accepted=warmup)
     e.set_state(0, ic.still_water_with_step(ic.gaussian_bathymetry(n, n, seed=3), n, dry_tol=1e-6))
     e.fill_ghosts()
     e.time_init(**TIME_KW)
@@ -234,6 +238,7 @@ def run_friction_batch(abi, ic, spg, t_final=2.0, n_inst=64):
     e.close()
     cell = (4.0 / nx) * (2.0 / ny)
     return dict(value=upd / wall, unit="cell-updates/s", instances=n_inst, wall_s=wall,
+                roofline_frac_wall_28B=upd / wall * 28.0 / 1e9 / HBM_PEAK_GBS,
                 steps_min=min(steps), steps_max=max(steps), mass_fp64_before=m0 * cell,
                 mass_fp64_after=m1 * cell, tuned=tuned)
 
@@ -295,6 +300,8 @@ def main():
                     help="skip the bathymetry (8192^2) and friction-batch legs")
     ap.add_argument("--profile-only", action="store_true",
                     help="FP32 fused run only (for rocprofv3 kernel traces and counters)")
+    ap.add_argument("--profile-bathymetry", action="store_true",
+                    help="bathymetry + wet/dry (config 3) run only, for rocprofv3 traces and counters")
     ap.add_argument("--profile-fp64", action="store_true",
                     help="FP64 fused (bit-exact) run only, for rocprofv3 kernel traces")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -322,6 +329,13 @@ def main():
 
     n = args.n
     cells = n * n
+    if args.profile_bathymetry:
+        log(f"bathymetry + wet/dry, FP32 {n}^2, {args.steps} accepted steps")
+        bat = run_bathymetry(abi, ic, n, args.steps, args.steps_per_graph, tiles=args.tiles,
+                             warmup=args.warmup)
+        print(json.dumps({k: (float(v) if hasattr(v, "dtype") else v) for k, v in bat.items()}),
+              flush=True)
+        return
     if args.profile_fp64:
         log(f"FP64 fused engine, dam break {n}^2, {args.steps} accepted steps")
         ef, rf = run_engine(abi, ic, n, abi.FP64, abi.FUSED, args.steps, args.warmup,
