@@ -25,10 +25,17 @@
 //     run a body without any of the range tests; the other rows, and the
 //     strips on the domain's xi boundaries, run the general body.
 //   * Divisions are reciprocals, square roots are v_sqrt_f32 (1 ulp).
+//     Without metric arrays dt/d is folded into the correction
+//     coefficients, which saves one instruction per interface and sweep.
 //   * A cell with h <= 0 (or NaN) in a sweep's range is reported like the
 //     reference's NaN-in-Jacobian / complex-eigenvalue stop
 //     (SMOD_jakobian.f90:55-58, SMOD_eigenvalue_problem.f90:72-83), unless
-//     the wet/dry cut-off is enabled.
+//     the wet/dry cut-off is enabled.  It is found from the sweep's speed
+//     maximum (such a cell's speed is +inf or NaN), with no test per cell.
+//   * The kernel is VALU-issue bound as much as HBM bound (VALUBusy ~99 % at
+//     ~0.87 of the bandwidth a plain 3-plane stream sustains), so
+//     instruction count matters: 142 VALU instructions per row in the
+//     unconditional body.
 //
 // Reference stages fused here (src/solver/MOD_solve.f90:23-175 and callees):
 // wave_decomposition/HLL (SMOD_approximate_Riemann_solvers.f90:60-101),
@@ -47,11 +54,15 @@ namespace f32 {
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float vsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// max of two speeds (non-negative floats, or NaN) as integers: one
-// instruction, and a NaN operand wins (it is reported as an error at the end
-// of the tile) instead of being dropped as by fmaxf
+// max of two speeds (non-negative floats, +inf or NaN) as unsigned integers:
+// one instruction, and +inf or a NaN of either sign wins instead of being
+// dropped as by fmaxf.  A cell with h <= 0 or NaN has a speed |u| + c that
+// is +inf or NaN (h = 0: u = hu * (1/0) is inf or NaN; h < 0: c = sqrt(g h)
+// is NaN), so a maximum whose bits reach those of +inf reports the cell
+// (kNotFinite) without a test per cell.
+constexpr unsigned kNotFinite = 0x7f800000u;
 __device__ __forceinline__ float smax2(float a, float b) {
-  return __builtin_bit_cast(float, max(__builtin_bit_cast(int, a), __builtin_bit_cast(int, b)));
+  return __builtin_bit_cast(float, max(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b)));
 }
 __device__ __forceinline__ float wave_smax(float v) {
   for (int o = 32; o >= 1; o >>= 1) v = smax2(v, __shfl_xor(v, o, 64));
@@ -123,7 +134,6 @@ struct Ctx {
   float gr[8];       // extrapolation ratios of the linear-extrapolation boundary
   float dtdx, dtdy, eps, g, dry, n2, gdx, gdy, dt;
   bool out_lane, xface, xlim;
-  unsigned long long xmask, ymask;  // lanes in the xi sweep's cell range / the output lanes
 };
 __device__ __forceinline__ float lft(const Ctx& C, float v) { return bperm(C.bl, v); }
 __device__ __forceinline__ float rgt(const Ctx& C, float v) { return bperm(C.br, v); }
@@ -143,9 +153,10 @@ __device__ __forceinline__ float ghost(int bc, float near, float far, int kk, bo
   return normal ? -v : v;
 }
 
+// per-lane maxima of the interface speeds; smy over all lanes, masked to the
+// output lanes at the end of the tile
 struct Red {
   float smx, smy;
-  unsigned long long badx, bady;  // ballots of lanes that met h <= 0
 };
 
 // eta-direction (DIR 2) or xi-direction (DIR 1) speed and flux of one cell
@@ -179,22 +190,24 @@ __device__ __forceinline__ void cell(float h, float hu, float hv, float g, float
 // The eps q terms (below FP32 resolution next to the others, and exactly
 // what the reference leaks into waves of a quiescent state) are dropped, so
 // a state at rest has exactly zero waves, and then w1 + w2 = dq * 2 s2 /
-// (2 s2 - eps), which is dq to FP32 resolution: w2 = dq - w1.  Waves are
-// zero when s2 <= eps, through a zero reciprocal: no branch.  The speed is
-// returned as it is: at s2 <= eps (1e-10) its waves are zero, and its share
-// of the CFL maximum is below any live interface's.
+// (2 s2 - eps), which is dq to FP32 resolution: w2 = dq - w1.
+// DRY: the state may hold dry cells (speed 0), so waves are zero when
+// s2 <= eps, through a zero reciprocal: no branch; the speed is returned as
+// it is, and at s2 <= eps (1e-10) its share of the CFL maximum is below any
+// live interface's.  Without dry cells s2 <= eps needs h < 1e-21 on both
+// sides, and the waves are the formula's, as in the reference.
+template <bool DRY>
 __device__ __forceinline__ float hll(const float qL[3], const float fL[3], float sL,
                                      const float qR[3], const float fR[3], float sR, float eps,
                                      float w1[3], float w2[3]) {
   const float s2 = smax2(sL, sR);
-  const bool live = s2 > eps;
+  const bool live = !DRY || s2 > eps;
   const float rD = live ? rcp(fmaf(-2.0f, s2, eps)) : 0.0f;
-  const float m = live ? 1.0f : 0.0f;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const float dq = qR[k] - qL[k], df = fR[k] - fL[k];
     w1[k] = fmaf(-s2, dq, df) * rD;
-    w2[k] = fmaf(dq, m, -w1[k]);
+    w2[k] = DRY ? fmaf(dq, live ? 1.0f : 0.0f, -w1[k]) : dq - w1[k];
   }
   return s2;
 }
@@ -268,7 +281,7 @@ __device__ __forceinline__ void source(float& h, float& hu, float& hv, int kind,
 template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
 __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const Slot& A,
                                        const Slot& B, float acc[3], float bm, float b0,
-                                       float bnew, float fin1[3], float finm[3]);
+                                       float bnew, float finm[3]);
 
 // Input rows are prefetched kPF rows ahead, through a ring of kPF buffers
 // (6 for the plain variants; 3 where metric arrays or source terms take the
@@ -292,34 +305,47 @@ __device__ __forceinline__ void xi_row(const Ctx& C, int r, const Pre& P, Red& R
     float s, f1, f2;
     if (wet) cell<1, true>(q[0], q[1], q[2], C.g, C.dry, s, f1, f2);
     else cell<1, false>(q[0], q[1], q[2], C.g, C.dry, s, f1, f2);
-    if (!wet) {
-      const unsigned long long b = __ballot(!(q[0] > 0.0f));
-      R.badx |= GEN ? (b & C.xmask) : b;
-    }
     const float qL[3] = {lft(C, q[0]), lft(C, q[1]), lft(C, q[2])};
     const float fL[3] = {qL[1], lft(C, f1), lft(C, f2)};
     const float fR[3] = {q[1], f1, f2};
     const float sL = lft(C, s);
     float w1[3], w2[3];
-    const float s2 = hll(qL, fL, sL, q, fR, s, C.eps, w1, w2);
+    const float s2 = hll<PHYS>(qL, fL, sL, q, fR, s, C.eps, w1, w2);
     R.smx = smax2(R.smx, (!GEN || C.xface) ? s2 : 0.0f);
+    float p1 = 0.5f, p2 = 0.5f;  // halved limiter values (unlimited: 1 / 2)
+    if (HO && LIM > 0) {
+      const float w1R[3] = {rgt(C, w1[0]), rgt(C, w1[1]), rgt(C, w1[2])};
+      const float w2L[3] = {lft(C, w2[0]), lft(C, w2[1]), lft(C, w2[2])};
+      p1 = half_phi<LIM>(w1, w1R, C.eps);
+      p2 = half_phi<LIM>(w2, w2L, C.eps);
+      if (GEN) {
+        p1 = C.xlim ? p1 : 0.5f;
+        p2 = C.xlim ? p2 : 0.5f;
+      }
+    }
+    if (HO && !MET) {
+      // dt/dxi folded into the coefficients: nu = s dt/dxi, the correction
+      // coefficients c = nu (1 - nu) p, and per field
+      //   dt/dxi H = (c1 - nu) w1 + c2 w2     (to the interface's low cell)
+      //   q - dt/dxi G = q + c1 w1 - (nu - c2) w2
+      // (the waves die before the shifted values arrive)
+      const float nu = C.dtdx * s2, cf = fmaf(-nu, nu, nu);
+      const float c1 = cf * p1, c2 = cf * p2, e = c1 - nu, am = nu - c2;
+      float loc[3], hr[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        loc[k] = fmaf(-am, w2[k], fmaf(c1, w1[k], q[k]));
+        hr[k] = rgt(C, fmaf(e, w1[k], c2 * w2[k]));
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) qx[k] = loc[k] - hr[k];
+      return;
+    }
     float F[3] = {0.0f, 0.0f, 0.0f};
     const float kap_l = MET ? shr1(mk) : 1.0f;
     if (HO) {
-      float p1 = 0.5f, p2 = 0.5f;  // halved limiter values (unlimited: 1 / 2)
-      if (LIM > 0) {
-        const float w1R[3] = {rgt(C, w1[0]), rgt(C, w1[1]), rgt(C, w1[2])};
-        const float w2L[3] = {lft(C, w2[0]), lft(C, w2[1]), lft(C, w2[2])};
-        p1 = half_phi<LIM>(w1, w1R, C.eps);
-        p2 = half_phi<LIM>(w2, w2L, C.eps);
-        if (GEN) {
-          p1 = C.xlim ? p1 : 0.5f;
-          p2 = C.xlim ? p2 : 0.5f;
-        }
-      }
-      // s (1 - s dt/dxi / kbar); kbar = 1 without metric arrays
-      const float coef = MET ? s2 * fmaf(-s2 * C.dtdx, rcp(0.5f * (kap_l + mk)), 1.0f)
-                             : s2 * fmaf(-C.dtdx, s2, 1.0f);
+      // s (1 - s dt/dxi / kbar)
+      const float coef = s2 * fmaf(-s2 * C.dtdx, rcp(0.5f * (kap_l + mk)), 1.0f);
       corr_flux(coef, p1, p2, w1, w2, F);
     }
     const float sg = MET ? mgx * s2 : s2;
@@ -355,14 +381,12 @@ __device__ __forceinline__ void prefetch(const Ctx& C, int r, Pre& P) {
 template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
 __device__ __forceinline__ void eta_row(const Ctx& C, int r, const float qx[3], float mk,
                                         float mgy, Slot& N, Slot& A, Slot& B, float acc[3],
-                                        Red& R, float& bm, float& b0, float bnew, float fin1[3],
-                                        float finm[3]) {
+                                        Red& R, float& bm, float& b0, float bnew, float finm[3]) {
   const bool wet = PHYS && C.wetdry;
 #pragma unroll
   for (int k = 0; k < 3; ++k) N.q[k] = qx[k];
   if (wet) cell<2, true>(qx[0], qx[1], qx[2], C.g, C.dry, N.s, N.f1, N.f2);
   else cell<2, false>(qx[0], qx[1], qx[2], C.g, C.dry, N.s, N.f1, N.f2);
-  if (!wet && (!GEN || r <= C.ny + 1)) R.bady |= __ballot(!(qx[0] > 0.0f)) & C.ymask;
   if (MET) {
     N.kap = mk;
     N.fac = C.dtdy * rcp(mk);
@@ -371,16 +395,15 @@ __device__ __forceinline__ void eta_row(const Ctx& C, int r, const float qx[3], 
   // interface r (rows r-1 | r)
   if (!GEN || r >= C.y0 - 1) {
     const float fA[3] = {A.q[2], A.f1, A.f2}, fN[3] = {qx[2], N.f1, N.f2};
-    N.s2 = hll(A.q, fA, A.s, qx, fN, N.s, C.eps, N.w1, N.w2);
-    const bool in = C.out_lane && (!GEN || (r >= 0 && r <= C.ny + 1));
-    R.smy = smax2(R.smy, in ? N.s2 : 0.0f);
+    N.s2 = hll<PHYS>(A.q, fA, A.s, qx, fN, N.s, C.eps, N.w1, N.w2);
+    R.smy = smax2(R.smy, (!GEN || (r >= 0 && r <= C.ny + 1)) ? N.s2 : 0.0f);
   } else {
     N.s2 = 0.0f;
 #pragma unroll
     for (int k = 0; k < 3; ++k) N.w1[k] = N.w2[k] = 0.0f;
   }
   // interface r-1: limiter, correction flux, G and H
-  if (!GEN || r >= C.y0 + 1) finish<LIM, HO, MET, PHYS, GEN>(C, r, N, A, B, acc, bm, b0, bnew, fin1, finm);
+  if (!GEN || r >= C.y0 + 1) finish<LIM, HO, MET, PHYS, GEN>(C, r, N, A, B, acc, bm, b0, bnew, finm);
   if (PHYS) {
     bm = b0;
     b0 = bnew;
@@ -394,11 +417,14 @@ __device__ __forceinline__ void eta_row(const Ctx& C, int r, const float qx[3], 
 // kernel's chunk test).
 template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
 __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot& B, Pre& P,
-                                    float acc[3], Red& R, float& bm, float& b0, float fin1[3],
-                                    float finm[3]) {
+                                    float acc[3], Red& R, float& bm, float& b0, float finm[3]) {
   // rows are not interleaved by the scheduler: that would move the
   // prefetch loads to the end of a chunk and waste their head start
   __builtin_amdgcn_sched_barrier(0);
+  // nor is the row's arithmetic started earlier, in the previous row (the
+  // input is in registers long before): the empty asm, ordered after the
+  // previous row's stores, is where the row's input values come from
+  asm volatile("" : "+v"(P.h), "+v"(P.hu), "+v"(P.hv));
   const float mk = P.k, mgy = P.gy;
   float bnew = 0.0f;
   if (PHYS && C.has_b && r >= C.y0) bnew = bload(C.bi, C.lo, (r - C.rbase) * C.p4);  // b(i, r-1)
@@ -412,7 +438,7 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
     prefetch<MET, PHYS>(C, r, P);
     __builtin_amdgcn_sched_barrier(0);
   }
-  eta_row<LIM, HO, MET, PHYS, GEN>(C, r, qx, mk, mgy, N, A, B, acc, R, bm, b0, bnew, fin1, finm);
+  eta_row<LIM, HO, MET, PHYS, GEN>(C, r, qx, mk, mgy, N, A, B, acc, R, bm, b0, bnew, finm);
   if (MET) prefetch<MET, PHYS>(C, r, P);  // the eta sweep used this row's metric values
 }
 
@@ -422,31 +448,46 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
 template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
 __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const Slot& A,
                                        const Slot& B, float acc[3], float bm, float b0,
-                                       float bnew, float fin1[3], float finm[3]) {
-  float Fa[3] = {0.0f, 0.0f, 0.0f};
-  if (HO) {
-    float p1 = 0.5f, p2 = 0.5f;  // halved limiter values (unlimited: 1 / 2)
-    if (LIM > 0 && (!GEN || (r - 1 >= 1 && r - 1 <= C.ny))) {
-      p1 = half_phi<LIM>(A.w1, N.w1, C.eps);
-      p2 = half_phi<LIM>(A.w2, B.w2, C.eps);
-    }
-    const float coef = MET ? A.s2 * fmaf(-A.s2 * C.dtdy, rcp(0.5f * (B.kap + A.kap)), 1.0f)
-                           : A.s2 * fmaf(-C.dtdy, A.s2, 1.0f);
-    corr_flux(coef, p1, p2, A.w1, A.w2, Fa);
+                                       float bnew, float finm[3]) {
+  // dH: H of interface r-1 (to row r-2), times dt/deta where that is folded
+  // into the coefficients (PS); aq: row r-1's xi state minus dt/deta G of the
+  // same interface
+  constexpr bool PS = HO && !MET;
+  float dH[3], aq[3];
+  float p1 = 0.5f, p2 = 0.5f;  // halved limiter values (unlimited: 1 / 2)
+  if (HO && LIM > 0 && (!GEN || (r - 1 >= 1 && r - 1 <= C.ny))) {
+    p1 = half_phi<LIM>(A.w1, N.w1, C.eps);
+    p2 = half_phi<LIM>(A.w2, B.w2, C.eps);
   }
-  const float sg = MET ? A.gy * A.s2 : A.s2;
-  float G[3], H[3];
+  if (PS) {
+    // dt/deta folded into the coefficients, as in the xi sweep
+    const float nu = C.dtdy * A.s2, cf = fmaf(-nu, nu, nu);
+    const float c1 = cf * p1, c2 = cf * p2, e = c1 - nu, am = nu - c2;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    G[k] = fmaf(sg, A.w2[k], -Fa[k]);
-    H[k] = fmaf(-sg, A.w1[k], Fa[k]);
+    for (int k = 0; k < 3; ++k) {
+      dH[k] = fmaf(e, A.w1[k], c2 * A.w2[k]);
+      aq[k] = fmaf(-am, A.w2[k], fmaf(c1, A.w1[k], A.q[k]));
+    }
+  } else {
+    float Fa[3] = {0.0f, 0.0f, 0.0f};
+    if (HO) {
+      const float coef = A.s2 * fmaf(-A.s2 * C.dtdy, rcp(0.5f * (B.kap + A.kap)), 1.0f);
+      corr_flux(coef, p1, p2, A.w1, A.w2, Fa);
+    }
+    const float sg = MET ? A.gy * A.s2 : A.s2;
+    const float facA = MET ? A.fac : C.dtdy;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      dH[k] = fmaf(-sg, A.w1[k], Fa[k]);
+      aq[k] = fmaf(-facA, fmaf(sg, A.w2[k], -Fa[k]), A.q[k]);
+    }
   }
   if (!GEN || r >= C.y0 + 2) {  // finish row j = r-2
     const int j = r - 2;
     const float facB = MET ? B.fac : C.dtdy;
     float qn[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) qn[k] = fmaf(-facB, H[k], acc[k]);
+    for (int k = 0; k < 3; ++k) qn[k] = PS ? acc[k] - dH[k] : fmaf(-facB, dH[k], acc[k]);
     if (PHYS && (C.src != PCW_SRC_NONE || C.wetdry)) {
       const float bx = shl1(b0) - shr1(b0), by = bnew - bm;
       source(qn[0], qn[1], qn[2], C.src, C.dry, C.g, C.gdx, C.gdy, C.dt, C.n2, bx, by);
@@ -485,18 +526,15 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
                    4u * (unsigned)col_pos(1, C.nslot, C.nx + kk), sj);
         }
       }
-      // eta-boundary ghosts
-      if (j == 1) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) fin1[k] = qn[k];
-      }
+      // eta-boundary ghosts; finm holds the finished row j-1 (rows next to
+      // the domain's bottom and top rows are finished in the general body)
       if (j == 2 && C.y0 == 1) {
 #pragma unroll
         for (int kk = 1; kk <= 2; ++kk) {
           const int sg = (2 - kk - C.rbase) * C.p4;  // ghost row 1-kk
 #pragma unroll
           for (int k = 0; k < 3; ++k)
-            bstore(ghost(C.bc[2], fin1[k], qn[k], kk, k == 2, C.gr[3 + kk]), C.qo[k], C.so, sg);
+            bstore(ghost(C.bc[2], finm[k], qn[k], kk, k == 2, C.gr[3 + kk]), C.qo[k], C.so, sg);
         }
       }
       if (j == C.ny) {
@@ -513,9 +551,8 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
     }
   }
   // row r-1: xi-updated state minus the contribution of its lower interface
-  const float facA = MET ? A.fac : C.dtdy;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) acc[k] = fmaf(-facA, G[k], A.q[k]);
+  for (int k = 0; k < 3; ++k) acc[k] = aq[k];
 }
 
 // Four waves per SIMD except with metric arrays: the source-term variant
@@ -591,8 +628,6 @@ k_step(FusedArgs<float> a) {
   }
   C.xface = lane >= 1 && C.i >= 0 && C.i <= C.nx + 1;  // interface i-1/2 in the xi CFL range
   C.xlim = C.i >= 1 && C.i <= C.nx;                   // interface limited
-  C.xmask = __ballot(C.i >= -1 && C.i <= C.nx + 1);
-  C.ymask = __ballot(C.out_lane);
   C.src = I.source;
   C.wetdry = a.wetdry;
   for (int k = 0; k < 4; ++k) C.bc[k] = I.bc[k];
@@ -609,7 +644,6 @@ k_step(FusedArgs<float> a) {
 
   Red R;
   R.smx = R.smy = 0.0f;
-  R.badx = R.bady = 0ull;
   Slot X, Y, Z;
   X.s2 = Y.s2 = Z.s2 = 0.0f;
   X.s = Y.s = Z.s = 0.0f;
@@ -621,7 +655,7 @@ k_step(FusedArgs<float> a) {
     X.w1[k] = Y.w1[k] = Z.w1[k] = X.w2[k] = Y.w2[k] = Z.w2[k] = 0.0f;
   }
   float acc[3] = {0.0f, 0.0f, 0.0f};
-  float fin1[3] = {0.0f, 0.0f, 0.0f}, finm[3] = {0.0f, 0.0f, 0.0f};
+  float finm[3] = {0.0f, 0.0f, 0.0f};
   float bm = 0.0f, b0 = 0.0f;
 
   // prefetch rows y0-2 .. y0-2+kPF-1 (clamped to y1+2: a tile of a two-row
@@ -661,31 +695,34 @@ k_step(FusedArgs<float> a) {
   int r = C.y0 - 2;
   while (r <= rl) {
     if (edge || r < rs || r + 5 > re) {
-      row<LIM, HO, MET, PHYS, true>(C, r, X, Y, Z, P[0 % kPF], acc, R, bm, b0, fin1, finm);
-      if (r + 1 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 1, Z, X, Y, P[1 % kPF], acc, R, bm, b0, fin1, finm);
-      if (r + 2 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 2, Y, Z, X, P[2 % kPF], acc, R, bm, b0, fin1, finm);
-      if (r + 3 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 3, X, Y, Z, P[3 % kPF], acc, R, bm, b0, fin1, finm);
-      if (r + 4 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 4, Z, X, Y, P[4 % kPF], acc, R, bm, b0, fin1, finm);
-      if (r + 5 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 5, Y, Z, X, P[5 % kPF], acc, R, bm, b0, fin1, finm);
+      row<LIM, HO, MET, PHYS, true>(C, r, X, Y, Z, P[0 % kPF], acc, R, bm, b0, finm);
+      if (r + 1 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 1, Z, X, Y, P[1 % kPF], acc, R, bm, b0, finm);
+      if (r + 2 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 2, Y, Z, X, P[2 % kPF], acc, R, bm, b0, finm);
+      if (r + 3 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 3, X, Y, Z, P[3 % kPF], acc, R, bm, b0, finm);
+      if (r + 4 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 4, Z, X, Y, P[4 % kPF], acc, R, bm, b0, finm);
+      if (r + 5 <= rl) row<LIM, HO, MET, PHYS, true>(C, r + 5, Y, Z, X, P[5 % kPF], acc, R, bm, b0, finm);
       r += 6;
       continue;
     }
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     do {
-      row<LIM, HO, MET, PHYS, false>(C, r, X, Y, Z, P[0 % kPF], acc, R, bm, b0, fin1, finm);
-      row<LIM, HO, MET, PHYS, false>(C, r + 1, Z, X, Y, P[1 % kPF], acc, R, bm, b0, fin1, finm);
-      row<LIM, HO, MET, PHYS, false>(C, r + 2, Y, Z, X, P[2 % kPF], acc, R, bm, b0, fin1, finm);
-      row<LIM, HO, MET, PHYS, false>(C, r + 3, X, Y, Z, P[3 % kPF], acc, R, bm, b0, fin1, finm);
-      row<LIM, HO, MET, PHYS, false>(C, r + 4, Z, X, Y, P[4 % kPF], acc, R, bm, b0, fin1, finm);
-      row<LIM, HO, MET, PHYS, false>(C, r + 5, Y, Z, X, P[5 % kPF], acc, R, bm, b0, fin1, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r, X, Y, Z, P[0 % kPF], acc, R, bm, b0, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 1, Z, X, Y, P[1 % kPF], acc, R, bm, b0, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 2, Y, Z, X, P[2 % kPF], acc, R, bm, b0, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 3, X, Y, Z, P[3 % kPF], acc, R, bm, b0, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 4, Z, X, Y, P[4 % kPF], acc, R, bm, b0, finm);
+      row<LIM, HO, MET, PHYS, false>(C, r + 5, Y, Z, X, P[5 % kPF], acc, R, bm, b0, finm);
       r += 6;
     } while (r + 5 <= re);
   }
 
   // wave maxima and error bits; the instance's last tile completes the attempt
-  const float mx = wave_smax(R.smx), my = wave_smax(R.smy);
-  tile_finish_slots(S, I, a.slots + (size_t)in * kSlots, tid - I.tile0, cur, mx, my,
-                    (R.badx != 0ull ? 1 : 0) | (R.bady != 0ull ? 2 : 0), lane);
+  const float mx = wave_smax(R.smx), my = wave_smax(C.out_lane ? R.smy : 0.0f);
+  const bool chk = !(PHYS && a.wetdry);  // with the wet/dry cut-off h <= 0 is a dry cell
+  const int bad = chk ? (__builtin_bit_cast(unsigned, mx) >= kNotFinite ? 1 : 0) |
+                            (__builtin_bit_cast(unsigned, my) >= kNotFinite ? 2 : 0)
+                      : 0;
+  tile_finish_slots(S, I, a.slots + (size_t)in * kSlots, tid - I.tile0, cur, mx, my, bad, lane);
 }
 
 template <bool HO, bool MET, bool PHYS>
