@@ -357,6 +357,23 @@ def main():
     kernel_ms = r32["avg_step_ms"]
     achieved = BYTES_PER_CELL_UPDATE_F32 * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
     traffic, traffic_tiles = load_traffic(n)
+    # the boundary's host transfers (start-up upload, snapshot download), timed
+    # once outside the timed region: the rate of a run that includes them
+    sync()
+    t0 = time.perf_counter()
+    qh = e32.get_state(0)
+    t_get = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e32.set_state(0, qh)
+    sync()
+    t_set = time.perf_counter() - t0
+    frame_gb = qh.nbytes / 1e9
+    del qh
+    host_transfers = {"frame_gb": frame_gb, "set_state_s": t_set, "get_state_s": t_get,
+                      "value_with_one_upload_and_download":
+                      r32["accepted"] * cells / (r32["wall_s"] + t_set + t_get)}
+    log(f"  host transfers of the {frame_gb:.2f} GB frame: set_state {t_set * 1e3:.1f} ms, "
+        f"get_state {t_get * 1e3:.1f} ms")
     e32.close()
     # what HBM sustains for the step's compulsory traffic: three planes of the
     # frame streamed in and three out, no stencil (pcw_stream_bandwidth)
@@ -371,7 +388,7 @@ def main():
                           "tuned": r32["tuned"], "steps": r32["accepted"]}), flush=True)
         return
 
-    out_extra = {"timed_runs_wall_s": r32["walls_s"]}
+    out_extra = {"timed_runs_wall_s": r32["walls_s"], "host_transfers": host_transfers}
     if n != 4096:
         log("FP32 fused engine, dam break 4096^2, same steps")
         e4, r4 = run_engine(abi, ic, 4096, abi.FP32, abi.FUSED, args.steps, args.warmup,
