@@ -426,6 +426,38 @@ def test_error_on_dry_cell(oracle, prec):
         assert ei.value.code == abi.PCW_ERR_STATE
 
 
+BAD_CELLS = {"dry": (0.0, 0.0, 0.0), "dry_moving": (0.0, 0.3, -0.2),
+             "negative": (-1e-3, 0.0, 0.0), "nan": (np.nan, 0.0, 0.0)}
+BAD_PLACES = {"interior_tile": (130, 128), "bottom_row": (1, 128), "first_column": (128, 1),
+              "last_column": (128, 256)}
+
+
+@pytest.mark.parametrize("place", list(BAD_PLACES))
+@pytest.mark.parametrize("kind", list(BAD_CELLS))
+def test_fp32_bad_cell_found_from_the_speed_maximum(kind, place):
+    """The FP32 step finds a cell with h <= 0 or NaN from its sweep's speed
+    maximum (the cell's |u| + sqrt(g h) is +inf or NaN) instead of a test per
+    cell; it reports PCW_ERR_STATE from the xi sweep wherever the cell is: in
+    a tile that runs the unconditional body (256^2 in 32-row tiles: strips at
+    columns 61-240, bands away from rows 1-2 and ny-1..ny), on the domain's
+    first row, or on its first or last column (the general body).  A good
+    state on the same tiles steps without error."""
+    abi = _abi()
+    n = 256
+    q = smooth_state(n, n, seed=2)
+    _say(f"fp32 bad cell {kind} at {place}")
+    with abi.Engine([abi.Domain(n, n)], precision=abi.FP32, limiter=4, tile_rows=32) as e:
+        e.set_state(0, q)
+        e.step(1e-4)
+        e.reject()
+        j, i = BAD_PLACES[place]
+        q[j + 1, i + 1] = BAD_CELLS[kind]
+        e.set_state(0, q)
+        with pytest.raises(abi.PcwError) as ei:
+            e.step(1e-4)
+        assert ei.value.code == abi.PCW_ERR_STATE
+
+
 def test_fp64_zero_depth_with_momentum_reports_eta():
     """h = 0 with non-zero momenta: the xi Jacobian has infinite entries but
     no NaN, so the xi sweep itself does not stop; its speed is infinite (the
