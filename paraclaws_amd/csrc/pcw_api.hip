@@ -517,6 +517,15 @@ int ensure_full_tiles(pcw_handle* h) {
 // below a quarter of the grid.  Cell values do not depend on the tiling.
 constexpr int kMinRetileRows = 16;
 constexpr int kResidentWavesPerCU = 16;
+
+// compute units of the current device (the one the handle's memory is on)
+int cu_count() {
+  int dev = 0, cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+  return cu > 0 ? cu : 1;
+}
+
 int maybe_retile(pcw_handle* h) {
   if (h->p.engine != PCW_ENGINE_FUSED || h->n < 2) return PCW_OK;
   std::vector<char> run(h->n, 0);
@@ -526,9 +535,7 @@ int maybe_retile(pcw_handle* h) {
     nrun += run[k];
   }
   if (nrun == 0 || nrun == h->tiled_running) return PCW_OK;
-  int cu = 0;
-  (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, 0);
-  const long long target = (long long)std::max(cu, 1) * kResidentWavesPerCU;
+  const long long target = (long long)cu_count() * kResidentWavesPerCU;
   int ty = h->tile_rows;
   while (ty > kMinRetileRows && count_tiles(h, ty, &run) < target)
     ty = std::max(kMinRetileRows, ty * 3 / 4);
@@ -1085,8 +1092,7 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
   // last round is full.
   const int wpb_c[] = {1, 2, 4};
   std::vector<std::pair<int, int>> cand;
-  int cu = 0;
-  (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, 0);
+  const int cu = cu_count();
   bool any_src = h->p.dry_tol > 0.0, any_nosrc = !(h->p.dry_tol > 0.0);
   for (const auto& d : h->dom) {
     any_src = any_src || d.source != PCW_SRC_NONE;
