@@ -1,0 +1,183 @@
+"""The batched workloads at their full sizes, and the mass budget of outflow
+boundaries.
+
+* The 512-instance dam-break ensemble (sizes 64^2..1024^2, each instance its
+  own adaptive dt) to t = 0.1: every instance ends at t = 0.1, and members run
+  alone give the batch's state bit for bit.  (The depth sum is not kept
+  exactly even with walls: the eta sweep reads the ghost rows as they were
+  before the step, src/main.f90:246, while the rows next to them are already
+  xi-updated, so a wall face passes a small flux; about 7e-4 of the depth sum
+  over this run.)
+* The Manning + bathymetry batch of 2048 x 1024 instances: members run alone
+  equal the batch bit for bit.
+* Outflow (zero-gradient) sides: in conservation form the xi fluxes
+  telescope, a zero-gradient face has zero waves, walls pass no mass and the
+  bed-slope and friction terms leave h alone, so one step changes the depth
+  sum by -dt/dxi * sum_j (hu(nx, j) - hu(1, j)) of the step's input state,
+  up to the reference's eps leak (SMOD_approximate_Riemann_solvers.f90:60-101:
+  s1 W1 + s2 W2 = df - eps q*, about 1e-10 per face), as long as the rows
+  next to the y-walls stay at rest (else the stale ghost rows above pass a
+  flux).  Checked step by step.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+T_ENS = 0.1
+TIME_KW = dict(t_final=T_ENS, dt0=1e-10, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10, growth=1.2,
+               shrink=0.5, n_snapshots=10)
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _ensemble_engine(abi, ic, members):
+    doms = [abi.Domain(n, n, bc=(4, 4, 4, 4)) for n, _ in members]
+    e = abi.Engine(doms, precision=abi.FP32, limiter=4)
+    for k, (n, r) in enumerate(members):
+        e.set_state(k, ic.dam_break(n, n, h_in=r))
+    e.fill_ghosts()
+    e.time_init(**TIME_KW)
+    return e
+
+
+def test_config4_ensemble_full_size():
+    from paraclaws_amd import abi, ic
+
+    members = ic.ensemble_config4()
+    assert len(members) == 512
+    _say("512-instance ensemble to t = 0.1")
+    e = _ensemble_engine(abi, ic, members)
+    m0 = np.array([e.totals(k)[0] for k in range(len(members))])
+    e.run(steps_per_graph=64)
+    steps = []
+    for k in range(len(members)):
+        st = e.status(k)
+        assert st.status == abi.STATUS_DONE and st.error == 0, (k, st.status, st.error)
+        assert st.time == pytest.approx(T_ENS, abs=1e-12), (k, st.time)
+        steps.append(st.accepted)
+    m1 = np.array([e.totals(k)[0] for k in range(len(members))])
+    _say(f"  accepted steps {min(steps)}..{max(steps)}; depth sum changed by at most "
+         f"{np.max(np.abs(m1 - m0) / m0):.2e} (stale ghost rows at the walls)")
+    ns = np.array([n for n, _ in members])
+    picks = [int(np.argmin(ns)), int(np.argmax(ns)), int(np.argsort(ns)[len(ns) // 2])]
+    batch = {k: e.get_state(k) for k in picks}
+    e.close()
+    for k in picks:
+        n, r = members[k]
+        _say(f"  member {k} ({n}^2, ratio {r:.3f}) alone")
+        with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=abi.FP32, limiter=4) as s:
+            s.set_state(0, ic.dam_break(n, n, h_in=r))
+            s.fill_ghosts()
+            s.time_init(**TIME_KW)
+            s.run(steps_per_graph=64)
+            st = s.status(0)
+            assert st.status == abi.STATUS_DONE and st.accepted == steps[k]
+            assert np.array_equal(s.get_state(0).view(np.uint64), batch[k].view(np.uint64))
+
+
+def _friction_member(abi, ic, k, nx, ny):
+    g = ic.SplitMix64(5)
+    nman = [g.uniform(0.01, 0.05) for _ in range(k + 1)][k]
+    dom = abi.Domain(nx, ny, bc=(2, 2, 4, 4), xi=(-2.0, 2.0), eta=(-1.0, 1.0),
+                     source=abi.SRC_BATHY_FRICTION, manning_n=nman)
+    b = ic.gaussian_bathymetry(nx, ny, seed=5000 + k, xi=(-2.0, 2.0), eta=(-1.0, 1.0))
+    q = ic.still_water_with_step(b, nx, xi=(-2.0, 2.0), dry_tol=1e-6)
+    return dom, b, q
+
+
+def test_config5_members_equal_alone():
+    """Four instances of the friction batch (2048 x 1024, Manning n from the
+    batch's stream, bathymetry seeds 5000+k, superbee, outflow in x, walls in
+    y) to t = 0.02 in one batch, and each alone: bit for bit."""
+    from paraclaws_amd import abi, ic
+
+    nx, ny, n_inst = 2048, 1024, 4
+    kw = dict(TIME_KW, t_final=0.02)
+    mem = [_friction_member(abi, ic, k, nx, ny) for k in range(n_inst)]
+    _say("friction batch, 4 x 2048x1024 to t = 0.02")
+    with abi.Engine([m[0] for m in mem], precision=abi.FP32, limiter=2, dry_tol=1e-6) as e:
+        for k, (_, b, q) in enumerate(mem):
+            e.set_state(k, q)
+            e.set_bathymetry(k, b)
+        e.fill_ghosts()
+        e.time_init(**kw)
+        e.run(steps_per_graph=32)
+        sts = [e.status(k) for k in range(n_inst)]
+        batch = [e.get_state(k) for k in range(n_inst)]
+    for k, (dom, b, q) in enumerate(mem):
+        assert sts[k].status == abi.STATUS_DONE and sts[k].error == 0
+        with abi.Engine([dom], precision=abi.FP32, limiter=2, dry_tol=1e-6) as s:
+            s.set_state(0, q)
+            s.set_bathymetry(0, b)
+            s.fill_ghosts()
+            s.time_init(**kw)
+            s.run(steps_per_graph=32)
+            assert s.status(0).accepted == sts[k].accepted
+            assert np.array_equal(s.get_state(0).view(np.uint64), batch[k].view(np.uint64)), k
+
+
+BUDGET_MODES = [("fp64", "fused"), ("fp64", "staged"), ("fp32", "fused")]
+
+
+@pytest.mark.parametrize("prec,engine", BUDGET_MODES, ids=[f"{p}-{e}" for p, e in BUDGET_MODES])
+def test_outflow_mass_budget(prec, engine):
+    """A config-5 style instance at 256 x 128 on [-2,2]x[-1,1] (Manning
+    n = 0.03, superbee, outflow x-sides, walls in y): bathymetry (b <= 0.6, so
+    that every cell stays wet), a dam-break step h += 1 on x < 0 and a flow
+    u = 0.5 in the band |y| < 0.5, flat still water in the 32 rows next to
+    each wall.  Each of 10 steps changes
+    the depth sum by -dt/dxi * sum_j (hu(nx, j) - hu(1, j)) of its input state
+    (FP64: to 1e-9 of the change plus the eps leak; FP32: to the rounding of
+    the stored state), and the 12 rows next to each wall stay at rest (a
+    disturbance spreads at most two rows per step)."""
+    from paraclaws_amd import abi, ic
+
+    nx, ny = 256, 128
+    xi = (-2.0, 2.0)
+    dxi = (xi[1] - xi[0]) / nx
+    band = np.zeros(ny + 4, dtype=bool)
+    band[2 + 32:2 + ny - 32] = True  # rows 33 .. ny-32: |y| < 0.5
+    b = ic.gaussian_bathymetry(nx, ny, seed=5001, xi=xi, eta=(-1.0, 1.0))
+    b *= 0.6 / b.max()
+    b[~band] = 0.0
+    q = ic.still_water_with_step(b, nx, xi=xi, step=0.0, dry_tol=1e-6)
+    x = ic.centres(nx + 4, xi[0] - 2 * dxi, xi[1] + 2 * dxi)
+    q[np.ix_(band, x < 0.0, [0])] += 1.0  # a dam-break step inside the band
+    q[band, :, 1] = 0.5 * q[band, :, 0]
+    p = abi.FP64 if prec == "fp64" else abi.FP32
+    eng = abi.STAGED if engine == "staged" else abi.FUSED
+    dom = abi.Domain(nx, ny, bc=(2, 2, 4, 4), xi=xi, eta=(-1.0, 1.0),
+                     source=abi.SRC_BATHY_FRICTION, manning_n=0.03)
+    _say(f"outflow budget {prec} {engine}")
+    worst = 0.0
+    with abi.Engine([dom], precision=p, engine=eng, limiter=2, dry_tol=1e-6) as e:
+        e.set_state(0, q)
+        e.set_bathymetry(0, b)
+        e.fill_ghosts()
+        quiet0 = e.get_state(0)[np.r_[2:14, ny - 10:ny + 2]]
+        for k in range(10):
+            s = e.get_state(0)[2:-2, 2:-2]
+            assert s[..., 0].min() > 0.1  # wet: no cut-off changes h
+            flux = float(np.sum(s[:, -1, 1]) - np.sum(s[:, 0, 1]))
+            speed = np.max(np.abs(s[..., 1:]).max(axis=-1) / s[..., 0] + np.sqrt(9.81 * s[..., 0]))
+            dt = 0.45 * dxi / speed
+            m0 = e.totals(0)[0]
+            e.step(dt)
+            e.accept()
+            m1 = e.totals(0)[0]
+            expect = -dt / dxi * flux
+            leak = 2.0 * (nx + 1) * ny * 1e-10 * 3.0 * dt / dxi  # eps q* per face, both sweeps
+            tol = 1e-9 * abs(expect) + leak if prec == "fp64" else 2e-4
+            assert abs((m1 - m0) - expect) <= tol, (k, m1 - m0, expect)
+            worst = max(worst, abs((m1 - m0) - expect))
+            assert abs(expect) > 0.5  # the budget is not trivially zero
+        quiet1 = e.get_state(0)[np.r_[2:14, ny - 10:ny + 2]]
+        # at rest up to the reference's eps leak of a state at rest (waves of
+        # eps q / (2 s), about 1e-11; the FP32 engine drops those terms)
+        assert np.abs(quiet1 - quiet0).max() <= 1e-9
+    _say(f"  worst |dM - budget| = {worst:.3e}")
