@@ -267,6 +267,105 @@ def cpu_baseline(n, target_s=15.0):
                        f"FP64 CPU restatement with OpenMP ({el:.1f} s)")
 
 
+def cpu_config1(abi, ic):
+    """Config 1, the reference's own CPU case (256^2 dam break, MC, walls,
+    CFL 0.9, to t = 0.5): the FP64 CPU restatement (OpenMP) timed over the
+    whole run, and beside it the GPU engines over the same run, the FP64
+    fused engine's final state compared bit for bit with the restatement's."""
+    from oracle import oracle as O
+
+    n = 256
+    cores = int(os.environ.get("PCW_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    cfg = O.Config(nx=n, ny=n, limiter=4, bc=(4, 4, 4, 4), nthreads=cores)
+    ts = O.TimeSettings(t_final=0.5, dt0=1e-10, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10,
+                        growth=1.2, shrink=0.5, n_snap=10)
+    q = O.dam_break(cfg)
+    q0 = q.copy()
+    t0 = time.perf_counter()
+    recs, tc = O.run(cfg, ts, q)
+    el = time.perf_counter() - t0
+    acc = sum(r[0] for r in recs)
+    out = {"cells": n * n, "accepted": acc, "attempts": len(recs), "t_final": tc.time,
+           "cpu": {"value": acc * n * n / el, "wall_s": el, "cores": cores, "kind": "port"}}
+    dts = [r[2] for r in recs if r[0] == 1]
+    for name, prec in (("gpu_fp64_fused", abi.FP64), ("gpu_fp32_fused", abi.FP32)):
+        with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=prec, limiter=4) as e:
+            e.set_state(0, q0)
+            e.fill_ghosts()
+            e.time_init(**TIME_KW)
+            tuned = e.autotune(steps=3)
+            walls = []
+            for _ in range(3):
+                e.set_state(0, q0)
+                e.fill_ghosts()
+                e.time_init(**TIME_KW)
+                sync()
+                t0 = time.perf_counter()
+                e.run(steps_per_graph=64)
+                sync()
+                walls.append(time.perf_counter() - t0)
+            st = e.status(0)
+            got = e.get_state(0)
+        rec = {"value": st.accepted * n * n / min(walls), "wall_s": min(walls),
+               "accepted": st.accepted, "tuned": tuned}
+        if prec == abi.FP64:
+            rec["equals_cpu_bitwise"] = bool(np.array_equal(got[2:-2, 2:-2], q[2:-2, 2:-2]))
+        else:
+            # the CPU run's accepted dt sequence replayed (the same times; a
+            # free FP32 run takes its own dt sequence)
+            with abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4))], precision=prec, limiter=4) as e:
+                e.set_state(0, q0)
+                e.fill_ghosts()
+                for dt in dts:
+                    e.step(dt)
+                    e.accept()
+                rec["errors_vs_fp64_same_dts"] = ic.field_errors(e.get_state(0), q)
+        out[name] = rec
+    return out
+
+
+def run_mapped(abi, ic, n, steps, spg):
+    """The reference's stretched example at n x n (x = 4 xi, y = 2 eta on
+    [-1,1]^2: kappa = 8, gamma_xi = 2, gamma_eta = 4, given as arrays), dam
+    break of radius 0.5 in physical space, walls, MC, FP32 metric-array kernel:
+    36 algorithmic bytes per cell-update (state in and out, three metric
+    planes in)."""
+    dom = abi.Domain(n, n, bc=(4, 4, 4, 4))
+    e = abi.Engine([dom], precision=abi.FP32, limiter=4)
+    c = ic.centres(n, -1.0, 1.0)  # frame columns -1 .. n+2
+    r = np.hypot(4.0 * c[None, :], 2.0 * c[:, None])
+    q = np.zeros((n + 4, n + 4, 3))
+    q[..., 0] = np.where(r < 0.5, 2.0, 1.0)
+    del r
+    shape = (n + 4, n + 4)
+    e.set_metrics(0, np.full(shape, 8.0), np.full(shape, 2.0), np.full(shape, 4.0))
+    e.set_state(0, q)
+    e.fill_ghosts()
+    e.time_init(**TIME_KW)
+    tuned = e.autotune(steps=3)
+    e.set_state(0, q)
+    e.fill_ghosts()
+    e.time_init(**TIME_KW)
+    e.run(steps_per_graph=spg, max_accepted=10)
+    e.set_state(0, q)
+    e.fill_ghosts()
+    e.time_init(**TIME_KW)
+    del q
+    sync()
+    t0 = time.perf_counter()
+    e.run(steps_per_graph=spg, max_accepted=steps)
+    sync()
+    wall = time.perf_counter() - t0
+    st = e.status(0)
+    avg_ms, _ = e.kernel_stats()
+    e.close()
+    cells = n * n
+    return dict(value=st.accepted * cells / wall, unit="cell-updates/s", steps=st.accepted,
+                rejected=st.rejected, kernel_avg_ms=avg_ms,
+                roofline_frac_36B=(36.0 * cells / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if avg_ms else None,
+                tuned=tuned)
+
+
 def load_traffic(n):
     """HBM bytes per launch of the fused FP32 step from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE runs, if they exist for this grid size and kernel,
@@ -471,12 +570,20 @@ def main():
         fr = run_friction_batch(abi, ic, 64)
         log(f"  {fr}")
         out_extra["friction_batch"] = fr
+        log(f"stretched grid (x = 4 xi, y = 2 eta), metric arrays, FP32 {n}^2, {args.steps} steps")
+        mp = run_mapped(abi, ic, n, args.steps, args.steps_per_graph)
+        log(f"  {mp}")
+        out_extra["mapped_metric_arrays"] = mp
 
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (FP64 restatement, OpenMP)")
         cpu = cpu_baseline(n)
         log(f"  {cpu}")
+        log("config 1 (256^2 to t = 0.5): CPU restatement and GPU engines over the whole run")
+        c1 = cpu_config1(abi, ic)
+        log(f"  {c1}")
+        out_extra["config1_full_run"] = c1
 
     line = {
         "metric": METRIC,

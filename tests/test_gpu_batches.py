@@ -146,7 +146,7 @@ def test_outflow_mass_budget(prec, engine):
     b *= 0.6 / b.max()
     b[~band] = 0.0
     q = ic.still_water_with_step(b, nx, xi=xi, step=0.0, dry_tol=1e-6)
-    x = ic.centres(nx + 4, xi[0] - 2 * dxi, xi[1] + 2 * dxi)
+    x = ic.centres(nx, *xi)  # frame columns -1 .. nx+2
     q[np.ix_(band, x < 0.0, [0])] += 1.0  # a dam-break step inside the band
     q[band, :, 1] = 0.5 * q[band, :, 0]
     p = abi.FP64 if prec == "fp64" else abi.FP32
