@@ -630,8 +630,18 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
   h->pool_elems = base;
   h->bpool_elems = bbase;
   // default tile heights: FP32 64 rows (memory-bound; the autotuner refines it), FP64
-  // 128 rows (arithmetic-bound: fewer recomputed halo rows per tile)
-  h->tile_rows = p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128);
+  // 128 rows (arithmetic-bound: fewer recomputed halo rows per tile).  Without a
+  // height from the caller, the bands are shortened (down to kMinRetileRows) while
+  // the tiles would not fill the GPU's resident waves once: a small domain stepped
+  // without the autotuner (the Fortran host, pcw_step) then still spreads over the
+  // CUs (256^2: 80-100 tiles instead of 10-20).  Cell values do not depend on it.
+  int ty = p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128);
+  if (p->tile_rows <= 1) {
+    const long long target = (long long)cu_count() * kResidentWavesPerCU;
+    while (ty > kMinRetileRows && count_tiles(h, ty) < target)
+      ty = std::max(kMinRetileRows, ty * 3 / 4);
+  }
+  h->tile_rows = ty;
   h->wpb = (p->waves_per_block >= 1 && p->waves_per_block <= 4) ? p->waves_per_block : 2;
   HIPCHK(hipStreamCreate(&h->stream));
   HIPCHK(hipEventCreate(&h->ev0));
