@@ -22,6 +22,8 @@
  *   pcw_run              -> the time loop of src/main.f90:193-283 with the
  *                          controller of src/time_control/MOD_time_control.f90
  *                          replicated on the device
+ *   pcw_stop_at_snapshots -> the loop's return to the host for snapshot
+ *                          output (src/main.f90:256-268)
  *
  * Return codes: 0 ok, 1 step rejected (solve_dimensional_split's ierr = -1),
  * negative = fatal (the host maps it to abort_program,
@@ -195,6 +197,12 @@ int pcw_read_log(pcw_handle* h, int32_t inst, pcw_step_record* out, int32_t cap,
  * the limit is raised (or 0). */
 int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts,
             int64_t max_accepted);
+/* on = 1: device-controlled runs stop each instance after the accepted step
+ * that reaches one of its snapshot times, so that the host can write the
+ * snapshot there (the snapshot branch of the time loop, src/main.f90:256-268);
+ * the next pcw_run continues the instance.  on = 0 (the default): runs go
+ * on through snapshot times. */
+int pcw_stop_at_snapshots(pcw_handle* h, int32_t on);
 int pcw_get_status(pcw_handle* h, int32_t inst, pcw_status* out);
 
 /* ---- measurement --------------------------------------------------------- */

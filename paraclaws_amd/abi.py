@@ -86,6 +86,7 @@ SIGNATURES = {
     "pcw_enable_log": ([_h, C.c_int32], C.c_int),
     "pcw_read_log": ([_h, C.c_int32, _P(PcwStepRecord), C.c_int32, _P(C.c_int32)], C.c_int),
     "pcw_run": ([_h, C.c_int32, C.c_int64, C.c_int64], C.c_int),
+    "pcw_stop_at_snapshots": ([_h, C.c_int32], C.c_int),
     "pcw_get_status": ([_h, C.c_int32, _P(PcwStatus)], C.c_int),
     "pcw_totals": ([_h, C.c_int32, _dp], C.c_int),
     "pcw_last_elapsed_ms": ([_h, _dp], C.c_int),
@@ -332,6 +333,9 @@ class Engine:
 
     def run(self, steps_per_graph=32, max_attempts=0, max_accepted=0):
         self._chk(self.L.pcw_run(self.h, steps_per_graph, max_attempts, max_accepted))
+
+    def stop_at_snapshots(self, on=True):
+        self._chk(self.L.pcw_stop_at_snapshots(self.h, 1 if on else 0))
 
     def status(self, k=0):
         s = PcwStatus()

@@ -1040,6 +1040,14 @@ int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts, int64_
   return check_errors(h);
 }
 
+int pcw_stop_at_snapshots(pcw_handle* h, int32_t on) {
+  if (!h) return PCW_ERR_ARG;
+  int rc = pull_states(h);
+  if (rc) return rc;
+  for (int k = 0; k < h->n; ++k) h->hst[k].stop_snap = on ? 1 : 0;
+  return push_states(h);
+}
+
 int pcw_get_status(pcw_handle* h, int32_t inst, pcw_status* out) {
   if (!h || !out || inst < 0 || inst >= h->n) return PCW_ERR_ARG;
   int rc = pull_states(h);

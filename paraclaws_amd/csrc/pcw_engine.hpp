@@ -92,6 +92,7 @@ struct InstState {
   int controlled;                     // 1: the last tile runs the controller
   int last_ok;                        // staged engine: 1 when the last attempt was kept
   int ghosts_due;                     // FP64 fused engine: 1 when the attempt just completed was accepted
+  int stop_snap;                      // 1: a device-controlled run stops the instance at each snapshot
   int log_n, log_cap;
   pcw_step_record* log;
 };
@@ -237,6 +238,24 @@ __host__ __device__ inline int finish_checks(InstState* S, double sx, double sy,
   }
   *ok = tc_attempt(S->tc, sx, sy, dxi, deta, rec);
   return 0;
+}
+
+// Bookkeeping after a device-controlled attempt (accept counters already
+// updated): the step log, then the instance's status.  It stops (DONE) when
+// the time loop of src/main.f90:193 ends (time >= time_final), at the
+// accepted-step limit of the run, or, when stop_snap is set, after the step
+// that reached a snapshot time (main.f90:256-268: the host writes the
+// snapshot, and the next run continues).
+__host__ __device__ inline void after_attempt(InstState* S, const pcw_step_record& rec) {
+  if (S->log && S->log_n < S->log_cap) S->log[S->log_n++] = rec;
+  if (S->tc.ended) {
+    S->status = PCW_STATUS_ERROR;
+    S->error = PCW_ERR_CONTROL;
+  } else if (!(S->tc.time < S->tc.time_final) ||
+             (S->max_accepted > 0 && S->accepted >= S->max_accepted) ||
+             (S->stop_snap && rec.accepted && rec.snapshot > 0)) {
+    S->status = PCW_STATUS_DONE;
+  }
 }
 
 }  // namespace pcw

@@ -346,14 +346,7 @@ __global__ void k_st_finish(const KInst* inst, InstState* st, int n) {
   S->last_ok = ok;
   if (ok) S->accepted += 1;
   else S->rejected += 1;
-  if (S->log && S->log_n < S->log_cap) S->log[S->log_n++] = rec;
-  if (S->tc.ended) {
-    S->status = PCW_STATUS_ERROR;
-    S->error = PCW_ERR_CONTROL;
-  } else if (!(S->tc.time < S->tc.time_final) ||
-             (S->max_accepted > 0 && S->accepted >= S->max_accepted)) {
-    S->status = PCW_STATUS_DONE;
-  }
+  after_attempt(S, rec);
 }
 
 // deterministic FP64 sums over the interior of buffer cur: pass 1

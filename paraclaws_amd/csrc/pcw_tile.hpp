@@ -34,14 +34,7 @@ __device__ __forceinline__ void finish_attempt(InstState* S, const KInst& I, int
   } else {
     S->rejected += 1;
   }
-  if (S->log && S->log_n < S->log_cap) S->log[S->log_n++] = rec;
-  if (S->tc.ended) {
-    S->status = PCW_STATUS_ERROR;
-    S->error = PCW_ERR_CONTROL;
-  } else if (!(S->tc.time < S->tc.time_final) ||
-             (S->max_accepted > 0 && S->accepted >= S->max_accepted)) {
-    S->status = PCW_STATUS_DONE;
-  }
+  after_attempt(S, rec);
 }
 
 // mx, my: the wave's maxima of the xi / eta interface speeds (already reduced

@@ -19,6 +19,7 @@ module MOD_solver_parameters
     ! device settings (custom.in): arithmetic and engine of the GPU path
     integer, protected :: device_precision = 0   ! 0 FP64 (reference order), 1 FP32
     integer, protected :: device_engine = 0      ! 0 fused step, 1 one kernel per stage
+    integer, protected :: device_loop = 0        ! 0 time loop on the host, 1 on the device between snapshots
 
 contains
 
@@ -36,10 +37,11 @@ contains
         limiter_method = limiter_method_
     end subroutine
 
-    subroutine set_device_parameters(precision_, engine_)
-        integer, intent(in) :: precision_, engine_
+    subroutine set_device_parameters(precision_, engine_, loop_)
+        integer, intent(in) :: precision_, engine_, loop_
         device_precision = precision_
         device_engine = engine_
+        device_loop = loop_
     end subroutine
 
 end module MOD_solver_parameters

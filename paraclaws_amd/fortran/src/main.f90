@@ -12,7 +12,9 @@ program paraCLAWs_gpu
     use MOD_IO
     use MOD_work_arrays, only: work_arrays
     use MOD_device, only: device_upload, device_download, device_commit, device_totals, &
-        device_source_kind
+        device_source_kind, device_loop_start, device_loop_segment, device_time, segment_attempts
+    use MOD_pcw, only: pcw_step_record
+    use MOD_solver_parameters, only: device_loop
     use MOD_error_handling, only: abort_program
     implicit none
 
@@ -25,8 +27,9 @@ program paraCLAWs_gpu
     double precision :: totals(3)
     integer :: n_accepted, n_rejected
     integer :: count0, count1, count_rate
-    integer :: i, j, ilow, ihigh, jlow, jhigh
-    logical :: state_hooks
+    integer :: i, j, k, ilow, ihigh, jlow, jhigh, nrec
+    logical :: state_hooks, snapshot_due
+    type(pcw_step_record), allocatable :: recs(:)
 
 1   format('paraCLAWs - time: ',f10.8,' current dt: ',f10.8,' CFL(max)=',f10.5,' next dt: ',f10.8,' => snapshot: ', i4)
 2   format('paraCLAWs - time: ',f10.8,' current dt: ',f10.8,' CFL(max)=',f10.5,' next dt: ',f10.8)
@@ -81,7 +84,60 @@ program paraCLAWs_gpu
 
     n_accepted = 0
     n_rejected = 0
+    if (device_loop == 1 .and. state_hooks) &
+        print *, "custom.in asks for the device time loop, but after_step needs the state " // &
+            "every step (host_state_hooks): the loop runs on the host"
     call system_clock(count0, count_rate)
+    if (device_loop == 1 .and. .not. state_hooks) then
+        ! The same loop with the steps on the device: graph-launched attempts
+        ! under the device's copy of the controller, back on the host at every
+        ! snapshot.  Each run's attempts are replayed through t_ctrl in order
+        ! (their CFL: the same controller decisions, dt sequence and terminal
+        ! output as the loop below), and the snapshot step is the run's last.
+        allocate(recs(segment_attempts))
+        call device_loop_start(t_ctrl)
+        do while (t_ctrl%time < t_ctrl%time_final)
+            call device_loop_segment(recs, nrec)
+            if (nrec == 0) call abort_program("device time loop: a run made no attempt")
+            do k = 1, nrec
+                call t_ctrl%update_max_CFL(recs(k)%cfl)
+                if (t_ctrl%check_timestep_validity() .neqv. (recs(k)%accepted /= 0)) &
+                    call abort_program("device time loop: the host controller disagrees with the device's")
+                if (recs(k)%accepted == 0) then
+                    print *, "***************************************************"
+                    print *, "WARNING: time step rejected!"
+                    call t_ctrl%updateTime(step_valid=.false.)
+                    call t_ctrl%reset_CFL()
+                    print *, "***************************************************"
+                    n_rejected = n_rejected + 1
+                    cycle
+                end if
+                call t_ctrl%updateTime(step_valid=.true.)
+                n_accepted = n_accepted + 1
+                snapshot_due = t_ctrl%time >= t_ctrl%snapshot_intervals(t_ctrl%get_snapshot_index())
+                if (snapshot_due .neqv. (recs(k)%snapshot > 0)) &
+                    call abort_program("device time loop: snapshot steps of host and device differ")
+                if (snapshot_due) then
+                    if (k /= nrec) call abort_program("device time loop: the run went past a snapshot")
+                    call device_download(solution%vars)
+                    call execute_post(decomposition, mesh, solution)
+                    call device_upload(solution%vars)     ! user_post's changes
+                    call writeData(decomposition, t_ctrl%time, solution)
+                    call write_vtk_rank(decomposition, mesh, solution, t_ctrl%time)
+                    call write_pvts_rank(decomposition, t_ctrl%time)
+                    write(*,1) t_ctrl%time, t_ctrl%get_previous_dt(), t_ctrl%get_CFL(), t_ctrl%get_dt(), &
+                        t_ctrl%get_snapshot_index()
+                    call t_ctrl%advance_snapshot_index()
+                else
+                    write(*,2) t_ctrl%time, t_ctrl%get_previous_dt(), t_ctrl%get_CFL(), t_ctrl%get_dt()
+                end if
+                call t_ctrl%reset_CFL()
+                call after_step(decomposition, t_ctrl, mesh, solution)
+            end do
+            if (device_time() /= t_ctrl%time) &
+                call abort_program("device time loop: device and host times differ")
+        end do
+    end if
     do while (t_ctrl%time < t_ctrl%time_final)
         call solve_dimensional_split(decomposition, t_ctrl, work, mesh, solution, ierr)
         if (ierr < 0) then

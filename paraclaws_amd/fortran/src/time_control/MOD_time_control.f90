@@ -40,6 +40,7 @@ module MOD_time_control
         procedure :: get_dt
         procedure :: get_previous_dt
         procedure :: get_snapshot_index
+        procedure :: get_limits
     end type time_control
 
 contains
@@ -192,5 +193,15 @@ contains
         integer :: v
         v = this%current_snapshot_index
     end function
+
+    subroutine get_limits(this, dt_max, dt_min, number_of_snapshots)
+        !! the private settings a copy of the controller on the device needs
+        class(time_control) :: this
+        double precision, intent(out) :: dt_max, dt_min
+        integer, intent(out) :: number_of_snapshots
+        dt_max = this%dt_max
+        dt_min = this%dt_min
+        number_of_snapshots = this%number_of_snapshots
+    end subroutine
 
 end module MOD_time_control

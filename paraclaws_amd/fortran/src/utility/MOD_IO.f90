@@ -79,19 +79,23 @@ contains
 
     subroutine read_custom_settings()
         !! custom.in: line 1 device precision (0 FP64, 1 FP32), line 2 engine
-        !! (0 fused step, 1 one kernel per reference stage).  Missing file or
-        !! lines keep the defaults (FP64, fused).
+        !! (0 fused step, 1 one kernel per reference stage), line 3 time loop
+        !! (0 on the host, step by step as the reference's; 1 on the device,
+        !! returning to the host at every snapshot).  Missing file or lines
+        !! keep the defaults (FP64, fused, host loop).
         use MOD_solver_parameters, only: set_device_parameters
-        integer :: filestat, p, e
+        integer :: filestat, p, e, l
         p = 0
         e = 0
+        l = 0
         open(unit=2, file='custom.in', status='old', action='read', iostat=filestat)
         if (filestat == 0) then
             read(2, *, iostat=filestat) p
             if (filestat == 0) read(2, *, iostat=filestat) e
+            if (filestat == 0) read(2, *, iostat=filestat) l
             close(2)
         end if
-        call set_device_parameters(p, e)
+        call set_device_parameters(p, e, l)
     end subroutine
 
     subroutine writeData(decomposition, time, solution)

@@ -9,6 +9,7 @@ module MOD_device
     type(c_ptr), save :: handle = c_null_ptr
     integer, save :: m_xi_ = 0, m_eta_ = 0
     integer, save :: device_source_kind = 0   ! the built-in source the engine applies
+    integer, parameter :: segment_attempts = 4096  ! attempts of one device-controlled run at most
 
 contains
 
@@ -110,6 +111,53 @@ contains
             call device_check(pcw_reject(handle), "pcw_reject")
         end if
     end subroutine
+
+    subroutine device_loop_start(t_ctrl)
+        !! The time loop on the device: a copy of the controller (pcw_time_init
+        !! with control.in's settings, at the initial time) that stops the run
+        !! after every step reaching a snapshot time.
+        use MOD_time_control, only: time_control
+        type(time_control), intent(inout) :: t_ctrl
+        type(pcw_time) :: t
+        double precision :: dt_max, dt_min
+        integer :: n_snapshots
+        call t_ctrl%get_limits(dt_max, dt_min, n_snapshots)
+        t%t0 = t_ctrl%time_initial
+        t%t_final = t_ctrl%time_final
+        t%dt0 = t_ctrl%get_dt()
+        t%cfl_target = t_ctrl%CFL_target
+        t%dt_max = dt_max
+        t%dt_min = dt_min
+        t%growth = t_ctrl%timestep_growth_limit
+        t%shrink = t_ctrl%timestep_shrink_limit
+        t%n_snapshots = n_snapshots
+        t%adaptive = merge(1, 0, t_ctrl%adaptive_timestepping)
+        call device_check(pcw_time_init(handle, t), "pcw_time_init")
+        call device_check(pcw_stop_at_snapshots(handle, 1_c_int32_t), "pcw_stop_at_snapshots")
+    end subroutine
+
+    subroutine device_loop_segment(recs, n)
+        !! One device-controlled run: graph-launched attempts until the run
+        !! stops (a snapshot or the final time reached) or segment_attempts
+        !! were made; recs(1:n) are its attempts in order.
+        type(pcw_step_record), intent(out) :: recs(segment_attempts)
+        integer, intent(out) :: n
+        integer(c_int32_t) :: m
+        call device_check(pcw_enable_log(handle, int(segment_attempts, c_int32_t)), "pcw_enable_log")
+        call device_check(pcw_run(handle, 32_c_int32_t, int(segment_attempts, c_int64_t), 0_c_int64_t), &
+            "pcw_run")
+        call device_check(pcw_read_log(handle, 0_c_int32_t, recs, int(segment_attempts, c_int32_t), m), &
+            "pcw_read_log")
+        n = m
+    end subroutine
+
+    function device_time() result(t)
+        !! simulated time of the device controller
+        double precision :: t
+        type(pcw_status) :: st
+        call device_check(pcw_get_status(handle, 0_c_int32_t, st), "pcw_get_status")
+        t = st%time
+    end function
 
     subroutine device_totals(out)
         double precision, intent(out) :: out(3)

@@ -1,7 +1,7 @@
 ! iso_c_binding interfaces of the gfx950 engine (include/pcw.h).
 !
 ! The derived types below are interoperable mirrors of pcw_params,
-! pcw_domain and pcw_time; every entry point returns the C status code
+! pcw_domain, pcw_time, pcw_step_record and pcw_status; every entry point returns the C status code
 ! (0 ok, 1 rejected, < 0 fatal).
 module MOD_pcw
     use iso_c_binding
@@ -27,6 +27,25 @@ module MOD_pcw
         integer(c_int32_t) :: higher_order, use_limiter, limiter
         integer(c_int32_t) :: tile_rows, waves_per_block, pad_
         real(c_double) :: grav, eps, dry_tol
+    end type
+
+    type, bind(C) :: pcw_time
+        real(c_double) :: t0, t_final, dt0, cfl_target, dt_max, dt_min, growth, shrink
+        integer(c_int32_t) :: n_snapshots
+        integer(c_int32_t) :: adaptive
+    end type
+
+    type, bind(C) :: pcw_step_record
+        integer(c_int32_t) :: accepted
+        integer(c_int32_t) :: snapshot
+        real(c_double) :: dt_used, cfl, time, dt_next
+    end type
+
+    type, bind(C) :: pcw_status
+        integer(c_int32_t) :: status
+        integer(c_int32_t) :: error
+        integer(c_int64_t) :: accepted, rejected
+        real(c_double) :: time, dt, cfl_last
     end type
 
     interface
@@ -115,6 +134,54 @@ module MOD_pcw
             type(c_ptr), value :: h
             integer(c_int32_t), value :: inst
             real(c_double), intent(out) :: out(3)
+            integer(c_int) :: rc
+        end function
+
+        function pcw_time_init(h, t) bind(C, name="pcw_time_init") result(rc)
+            import :: c_int, c_ptr, pcw_time
+            type(c_ptr), value :: h
+            type(pcw_time), intent(in) :: t
+            integer(c_int) :: rc
+        end function
+
+        function pcw_enable_log(h, capacity) bind(C, name="pcw_enable_log") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: capacity
+            integer(c_int) :: rc
+        end function
+
+        function pcw_read_log(h, inst, out, cap, n) bind(C, name="pcw_read_log") result(rc)
+            import :: c_int, c_int32_t, c_ptr, pcw_step_record
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: inst
+            type(pcw_step_record), intent(out) :: out(*)
+            integer(c_int32_t), value :: cap
+            integer(c_int32_t), intent(out) :: n
+            integer(c_int) :: rc
+        end function
+
+        function pcw_run(h, steps_per_graph, max_attempts, max_accepted) bind(C, name="pcw_run") &
+                result(rc)
+            import :: c_int, c_int32_t, c_int64_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: steps_per_graph
+            integer(c_int64_t), value :: max_attempts, max_accepted
+            integer(c_int) :: rc
+        end function
+
+        function pcw_stop_at_snapshots(h, on) bind(C, name="pcw_stop_at_snapshots") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: on
+            integer(c_int) :: rc
+        end function
+
+        function pcw_get_status(h, inst, out) bind(C, name="pcw_get_status") result(rc)
+            import :: c_int, c_int32_t, c_ptr, pcw_status
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: inst
+            type(pcw_status), intent(out) :: out
             integer(c_int) :: rc
         end function
     end interface

@@ -282,20 +282,24 @@ def test_fortran_host_bathymetry_example_matches_oracle(oracle, tmp_path, engine
 
 
 @pytest.mark.gpu
-def test_fortran_host_hooks_change_the_device_state(oracle, tmp_path):
+@pytest.mark.parametrize("loop", [0, 1])
+def test_fortran_host_hooks_change_the_device_state(oracle, tmp_path, loop):
     """user_post (through execute_post at start-up and at every snapshot) and
     after_step work on the current state and their changes reach the device:
     the test case's user_post raises h by 1e-3 in every interior cell, its
     after_step zeroes hu after the fifth accepted step.  The oracle replays
     the same run step by step with the same changes between steps; the final
-    states are equal bit for bit (FP64)."""
+    states are equal bit for bit (FP64).  Asked for the device time loop
+    (custom.in line 3 = 1), a case whose after_step needs the state every
+    step says so and runs the host loop."""
     exe = _exe(HOOKS_CASE)
     n, t_final = 64, 0.05
     (tmp_path / "control.in").write_text(_control_in(n, t_final))
-    (tmp_path / "custom.in").write_text("0\n0\n")
-    print("fortran host hooks", flush=True)
+    (tmp_path / "custom.in").write_text(f"0\n0\n{loop}\n")
+    print(f"fortran host hooks loop={loop}", flush=True)
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert ("the loop runs on the host" in r.stdout) == (loop == 1)
     got = np.fromfile(tmp_path / "output" / "final_state.bin", dtype=np.float64).reshape(n + 4, n + 4, 3)
     cfg = oracle.Config(nx=n, ny=n, limiter=4, bc=(4, 4, 4, 4))
     ts = oracle.TimeSettings(t_final=t_final, cfl_target=0.9, dt_max=1e-2, n_snap=10)
@@ -333,3 +337,56 @@ def test_fortran_host_rejects_a_nonzero_source_function(tmp_path):
     assert r.returncode != 0
     assert "source_function returns a nonzero source" in r.stdout + r.stderr
     assert not (tmp_path / "output" / "final_state.bin").exists()
+
+
+def _run_host(exe, case_dir, control, custom):
+    case_dir.mkdir()
+    (case_dir / "control.in").write_text(control)
+    (case_dir / "custom.in").write_text(custom)
+    r = subprocess.run([exe], cwd=case_dir, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    lines = r.stdout.splitlines()
+    first = next(k for k, ln in enumerate(lines) if "paraCLAWs - time:" in ln)
+    last = next(k for k, ln in enumerate(lines) if ln.startswith("accepted steps:"))
+    return lines[first:last], lines[last]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,dt0", [(0, 1e-10), (0, 0.2), (1, 1e-10)])
+def test_fortran_host_device_loop_equals_host_loop(oracle, tmp_path, precision, dt0):
+    """custom.in line 3 = 1 runs the time loop on the device (graph-launched
+    attempts under the device's copy of the controller, back on the host at
+    every snapshot, each run's attempts replayed through the host's
+    time_control).  Against the host loop (line 3 = 0) on the same case: the
+    same terminal output line for line (step lines, rejection messages and
+    snapshot lines), the same accepted and rejected counts, the same snapshot
+    directories and the same final state bit for bit; dt0 = 0.2 starts with a
+    rejected attempt (CFL far above the target).  FP64 with dt0 = 1e-10 is
+    also the oracle's run bit for bit."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran host not built (run __graft_entry__.build())")
+    n, t_final = 64, 0.05
+    control = _control_in(n, t_final, dt0=dt0)
+    print(f"fortran host loops precision={precision} dt0={dt0}", flush=True)
+    out_h, tail_h = _run_host(EXE, tmp_path / "host", control, f"{precision}\n0\n0\n")
+    out_d, tail_d = _run_host(EXE, tmp_path / "device", control, f"{precision}\n0\n1\n")
+    assert out_d == out_h
+    assert tail_d.split("wall:")[0] == tail_h.split("wall:")[0]
+    if dt0 > 1e-3:
+        assert any("WARNING: time step rejected!" in ln for ln in out_d)
+    got = {}
+    for mode in ("host", "device"):
+        got[mode] = np.fromfile(tmp_path / mode / "output" / "final_state.bin", dtype=np.float64)
+        snaps = sorted(x for x in os.listdir(tmp_path / mode / "output") if x[0].isdigit())
+        got[mode + "_snaps"] = snaps
+    assert got["device_snaps"] == got["host_snaps"] and len(got["host_snaps"]) == 11
+    assert np.array_equal(got["device"].view(np.uint64), got["host"].view(np.uint64))
+    if precision == 0 and dt0 < 1e-3:
+        cfg = oracle.Config(nx=n, ny=n, limiter=4, bc=(4, 4, 4, 4))
+        ts = oracle.TimeSettings(t_final=t_final, cfl_target=0.9, dt_max=1e-2, n_snap=10)
+        q = oracle.dam_break(cfg)
+        oracle.run(cfg, ts, q)
+        from tests.helpers import sides_mask
+
+        m = sides_mask(n, n)
+        assert np.array_equal(got["device"].reshape(n + 4, n + 4, 3)[m], q[m])
