@@ -102,31 +102,25 @@ contains
         use MOD_decomposition, only: decomp_info
         use MOD_domain, only: solution_fields
         use MOD_solver_parameters, only: num_ghost, num_equations, num_aux
+        use MOD_textio, only: write_e26_records
         type(decomp_info), intent(in) :: decomposition
         double precision, intent(in) :: time
         type(solution_fields), intent(in) :: solution
-        integer :: i, j, m, ilow, ihigh, jlow, jhigh
+        integer :: ilow, ihigh, jlow, jhigh, i0, i1, j0, j1
         character(len=40) :: timestamp_str, dirname, subdirname
-1001    format(50e26.16)
         call decomposition%get_local_block_bounds(ilow, ihigh, jlow, jhigh)
+        i0 = ilow - num_ghost
+        i1 = ihigh + num_ghost
+        j0 = jlow - num_ghost
+        j1 = jhigh + num_ghost
         write(timestamp_str, '(F20.10)') time
         dirname = 'output/'//trim(adjustl(timestamp_str))
         subdirname = trim(dirname)//'/rank_0'
         call execute_command_line('mkdir -p '//trim(subdirname))
-        open(unit=1, file=trim(subdirname)//'/qout.dat', status='unknown', form='formatted')
-        do j = jlow-num_ghost, jhigh+num_ghost
-            do i = ilow-num_ghost, ihigh+num_ghost
-                write(1,1001) (solution%vars(m,i,j), m=1,num_equations)
-            end do
-        end do
-        close(unit=1)
-        open(unit=2, file=trim(subdirname)//'/aout.dat', status='unknown', form='formatted')
-        do j = jlow-num_ghost, jhigh+num_ghost
-            do i = ilow-num_ghost, ihigh+num_ghost
-                write(2,1001) (solution%aux(m,i,j), m=1,num_aux)
-            end do
-        end do
-        close(unit=2)
+        ! qout.dat and aout.dat: one line of (50e26.16) fields per frame cell, j
+        ! outer, i inner, ghosts included
+        call write_e26_records(trim(subdirname)//'/qout.dat', solution%vars(1:num_equations, i0:i1, j0:j1))
+        call write_e26_records(trim(subdirname)//'/aout.dat', solution%aux(1:num_aux, i0:i1, j0:j1))
     end subroutine
 
     subroutine write_mesh_to_disk(decomposition, mesh)

@@ -113,7 +113,7 @@ end program
             if f.endswith(".o") and f != "main.o"]
     lib = os.path.join(ROOT, "paraclaws_amd")
     fc = "/opt/rocm/lib/llvm/bin/flang"
-    r = subprocess.run([fc, "-cpp", "-module-dir", str(tmp_path), "-I", os.path.join(build, "mod"),
+    r = subprocess.run([fc, "-cpp", "-fopenmp", "-module-dir", str(tmp_path), "-I", os.path.join(build, "mod"),
                         "t.f90", *objs, f"-L{lib}", "-lpcw_hip", f"-Wl,-rpath,{lib}", "-o", "t"],
                        cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
@@ -125,6 +125,67 @@ end program
         assert lines[n] == base64.b64encode(raw[:n]).decode()
     a = np.array([1.5, -2.25, 1e300], dtype="<f8").tobytes()
     assert lines[8] == (base64.b64encode(np.uint32(len(a)).tobytes()) + base64.b64encode(a)).decode()
+
+
+_TEXTIO_DRIVER = """
+program check_textio
+    use MOD_textio, only: write_e26_records
+    implicit none
+    integer, parameter :: ni = 37, nj = 150
+    double precision, allocatable :: a(:, :, :)
+    integer :: m, i, j, k, u
+    character(len=16) :: name
+    do m = 3, 10, 7
+        allocate(a(m, ni, nj))
+        call random_seed(put=[(977 + k, k = 1, 64)])
+        call random_number(a)
+        a = (a - 0.5d0) * 10.d0 ** (floor(a * 60) - 30)
+        a(1, 1, 1) = 0.d0
+        a(2, 1, 1) = -0.d0
+        a(3, 1, 1) = 4.9406564584124654d-324
+        a(1, 2, 1) = 1.7976931348623157d308
+        a(2, 2, 1) = 0.99999999999999995d0
+        a(3, 2, 1) = 1.d100
+        a(1, 3, 1) = -9.81000041961669921875d0
+        a(2, 3, 1) = transfer(int(z'7FF8000000000000', 8), 1.d0)
+        a(3, 3, 1) = -transfer(int(z'7FF0000000000000', 8), 1.d0)
+        write(name, '(a,i0)') 'serial_', m
+        open(newunit=u, file=trim(name), status='replace', form='formatted')
+        do j = 1, nj
+            do i = 1, ni
+                write(u, '(50e26.16)') a(:, i, j)
+            end do
+        end do
+        close(u)
+        write(name, '(a,i0)') 'parallel_', m
+        call write_e26_records(trim(name), a)
+        deallocate(a)
+    end do
+end program
+"""
+
+
+def test_snapshot_text_writer_equals_the_serial_write(tmp_path):
+    """MOD_textio.write_e26_records (qout.dat/aout.dat of every snapshot:
+    internal writes of (50e26.16) on OpenMP threads, blocks of rows streamed
+    to the file) writes exactly the bytes of the reference's serial
+    write(u, '(50e26.16)') a(:, i, j) loop: records of 3 and 10 fields, 150
+    rows (not a multiple of the 64-row blocks), values over 60 decades with
+    signed zeros, a denormal, the largest double, 3-digit exponents, NaN and
+    -Inf, on 4 threads."""
+    fc = "/opt/rocm/lib/llvm/bin/flang"
+    if not os.path.exists(fc):
+        pytest.skip("flang not found")
+    (tmp_path / "check.f90").write_text(_TEXTIO_DRIVER)
+    src = os.path.join(FDIR, "src", "utility", "MOD_textio.f90")
+    subprocess.run([fc, "-O2", "-fopenmp", "-o", "check", src, "check.f90"], cwd=tmp_path, check=True,
+                   capture_output=True)
+    env = dict(os.environ, OMP_NUM_THREADS="4")
+    subprocess.run(["./check"], cwd=tmp_path, check=True, env=env)
+    for m in (3, 10):
+        serial = (tmp_path / f"serial_{m}").read_bytes()
+        assert serial.count(b"\n") == 37 * 150
+        assert (tmp_path / f"parallel_{m}").read_bytes() == serial
 
 
 def _control_in(n, t_final, limiter=4, bc=(4, 4, 4, 4), cfl=0.9, dt0=1e-10, ho=True, lim=True):
