@@ -321,6 +321,45 @@ def cpu_config1(abi, ic):
                     e.accept()
                 rec["errors_vs_fp64_same_dts"] = ic.field_errors(e.get_state(0), q)
         out[name] = rec
+    out["fortran_host"] = fortran_host_config1(q)
+    return out
+
+
+FORTRAN_EXAMPLE = os.path.join(ROOT, "paraclaws_amd", "fortran", "examples", "dam_break")
+
+
+def fortran_host_config1(q_cpu):
+    """Config 1 through the Fortran host program (the drop-in: control.in of
+    examples/dam_break, 10 snapshots written in the reference's formats), FP64
+    fused engine, with the time loop on the host and on the device
+    (custom.in line 3); the wall time of the time loop as the program reports
+    it, and its final state against the CPU restatement's."""
+    import re
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(FORTRAN_EXAMPLE, "build", "paraclaws_gpu")
+    if not os.path.exists(exe):
+        return {"skipped": "Fortran host not built"}
+    out = {}
+    for loop, name in ((0, "host_loop"), (1, "device_loop")):
+        with tempfile.TemporaryDirectory() as d:
+            with open(os.path.join(FORTRAN_EXAMPLE, "control.in")) as f:
+                control = f.read()
+            with open(os.path.join(d, "control.in"), "w") as f:
+                f.write(control)
+            with open(os.path.join(d, "custom.in"), "w") as f:
+                f.write(f"0\n0\n{loop}\n")
+            r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=600)
+            m = re.search(r"accepted steps:\s*(\d+)\s+rejected:\s*(\d+)\s+wall:\s*([0-9.]+)", r.stdout)
+            if r.returncode != 0 or not m:
+                out[name] = {"failed": (r.stdout + r.stderr)[-500:]}
+                continue
+            got = np.fromfile(os.path.join(d, "output", "final_state.bin"), dtype=np.float64)
+            got = got.reshape(q_cpu.shape)
+            out[name] = {"wall_s": float(m.group(3)), "accepted": int(m.group(1)),
+                         "rejected": int(m.group(2)), "snapshots_written": 10,
+                         "equals_cpu_bitwise": bool(np.array_equal(got[2:-2, 2:-2], q_cpu[2:-2, 2:-2]))}
     return out
 
 
