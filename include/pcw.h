@@ -150,7 +150,9 @@ int pcw_set_metrics(pcw_handle* h, int32_t inst, const double* kappa, const doub
 int pcw_step(pcw_handle* h, const double* dt, double* smax);
 /* pcw_accept / pcw_reject apply to the instances pcw_step advanced: those
  * whose status is RUNNING (every instance after pcw_create; a device-
- * controlled run leaves finished ones DONE until pcw_time_init). */
+ * controlled run leaves finished ones DONE until pcw_time_init).  They only
+ * enqueue work on the handle's stream; the entry points that read results
+ * back (pcw_step, pcw_get_state, pcw_totals, ...) synchronise it. */
 int pcw_accept(pcw_handle* h);
 int pcw_reject(pcw_handle* h);
 /* Single sweeps in place on the current state (staged engine semantics). */

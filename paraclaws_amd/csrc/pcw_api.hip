@@ -821,10 +821,11 @@ int pcw_step(pcw_handle* h, const double* dt, double* smax) {
   return step_bad_code(h);
 }
 
+// Accept and reject only enqueue their work on the handle's stream: every
+// entry point that reads results back synchronises the stream first.
 int pcw_accept(pcw_handle* h) {
   if (!h) return PCW_ERR_ARG;
   if (h->p.engine == PCW_ENGINE_FUSED) HIPCHK(launch_flip(h->dst, h->n, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
   return PCW_OK;
 }
 
@@ -832,7 +833,6 @@ int pcw_reject(pcw_handle* h) {
   if (!h) return PCW_ERR_ARG;
   if (h->p.engine == PCW_ENGINE_STAGED)
     HIPCHK(launch_copy<double>((double*)h->pool, h->dinst, h->dst, h->n, 0, 0, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
   return PCW_OK;
 }
 

@@ -10,6 +10,10 @@ module MOD_device
     integer, save :: m_xi_ = 0, m_eta_ = 0
     integer, save :: device_source_kind = 0   ! the built-in source the engine applies
     integer, parameter :: segment_attempts = 4096  ! attempts of one device-controlled run at most
+    ! the device state's ghost cells are those of its interior: a step fills them
+    ! (every engine, for the new and the rolled-back state alike), an upload of a
+    ! host state does not
+    logical, save :: ghosts_current = .false.
 
 contains
 
@@ -62,6 +66,7 @@ contains
     subroutine device_upload(vars)
         double precision, intent(in), contiguous, target :: vars(:,:,:)
         call device_check(pcw_set_state(handle, 0_c_int32_t, vars), "pcw_set_state")
+        ghosts_current = .false.
     end subroutine
 
     subroutine device_download(vars)
@@ -83,7 +88,11 @@ contains
     end subroutine
 
     subroutine device_fill_ghosts()
+        !! the physical ghost cells from the interior; after a step they are
+        !! already filled, and the refill is skipped
+        if (ghosts_current) return
         call device_check(pcw_fill_ghosts(handle), "pcw_fill_ghosts")
+        ghosts_current = .true.
     end subroutine
 
     subroutine device_step(dt, smax_xi, smax_eta, eta_failed)
@@ -110,6 +119,7 @@ contains
         else
             call device_check(pcw_reject(handle), "pcw_reject")
         end if
+        ghosts_current = .true.
     end subroutine
 
     subroutine device_loop_start(t_ctrl)
@@ -146,6 +156,7 @@ contains
         call device_check(pcw_enable_log(handle, int(segment_attempts, c_int32_t)), "pcw_enable_log")
         call device_check(pcw_run(handle, 32_c_int32_t, int(segment_attempts, c_int64_t), 0_c_int64_t), &
             "pcw_run")
+        ghosts_current = .true.
         call device_check(pcw_read_log(handle, 0_c_int32_t, recs, int(segment_attempts, c_int32_t), m), &
             "pcw_read_log")
         n = m

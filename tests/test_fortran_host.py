@@ -310,22 +310,23 @@ def test_device_physics_override_replaces_the_base_submodules():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("engine", [0, 1])
-def test_fortran_host_bathymetry_example_matches_oracle(oracle, tmp_path, engine):
+@pytest.mark.parametrize("engine,loop", [(0, 0), (1, 0), (0, 1)])
+def test_fortran_host_bathymetry_example_matches_oracle(oracle, tmp_path, engine, loop):
     """Config 3's physics selected through the host: the example's
     SMOD_device_physics override (bathymetry slope, wet/dry cut-off) and
     initial state (still water over the seeded Gaussian bathymetry, dam-break
     step on x < 0, dry cells), outflow sides, run by the Fortran program on
-    the GPU in FP64, equal the oracle bit for bit."""
+    the GPU in FP64, equal the oracle bit for bit, with the time loop on the
+    host (both engines) and on the device (custom.in line 3)."""
     from paraclaws_amd import ic
 
     exe = _exe(BATHY_EXAMPLE)
     nx = ny = 96
     (tmp_path / "control.in").write_text(_control_in(nx, 0.03, bc=(2, 2, 2, 2)))
-    (tmp_path / "custom.in").write_text(f"0\n{engine}\n")
+    (tmp_path / "custom.in").write_text(f"0\n{engine}\n{loop}\n")
     b = ic.gaussian_bathymetry(nx, ny, seed=3)
     b.tofile(tmp_path / "bathymetry.bin")
-    print(f"fortran host bathymetry example engine={engine}", flush=True)
+    print(f"fortran host bathymetry example engine={engine} loop={loop}", flush=True)
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     assert "device source kind 1" in r.stdout
@@ -413,24 +414,25 @@ def _run_host(exe, case_dir, control, custom):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision,dt0", [(0, 1e-10), (0, 0.2), (1, 1e-10)])
-def test_fortran_host_device_loop_equals_host_loop(oracle, tmp_path, precision, dt0):
+@pytest.mark.parametrize("precision,engine,dt0", [(0, 0, 1e-10), (0, 0, 0.2), (1, 0, 1e-10), (0, 1, 0.2)])
+def test_fortran_host_device_loop_equals_host_loop(oracle, tmp_path, precision, engine, dt0):
     """custom.in line 3 = 1 runs the time loop on the device (graph-launched
     attempts under the device's copy of the controller, back on the host at
     every snapshot, each run's attempts replayed through the host's
     time_control).  Against the host loop (line 3 = 0) on the same case: the
     same terminal output line for line (step lines, rejection messages and
     snapshot lines), the same accepted and rejected counts, the same snapshot
-    directories and the same final state bit for bit; dt0 = 0.2 starts with a
-    rejected attempt (CFL far above the target).  FP64 with dt0 = 1e-10 is
-    also the oracle's run bit for bit."""
+    directories and the same final state bit for bit, for the fused engines
+    and the staged one (engine 1); dt0 = 0.2 starts with a rejected attempt
+    (CFL far above the target).  FP64 with dt0 = 1e-10 is also the oracle's
+    run bit for bit."""
     if not os.path.exists(EXE):
         pytest.skip("Fortran host not built (run __graft_entry__.build())")
     n, t_final = 64, 0.05
     control = _control_in(n, t_final, dt0=dt0)
-    print(f"fortran host loops precision={precision} dt0={dt0}", flush=True)
-    out_h, tail_h = _run_host(EXE, tmp_path / "host", control, f"{precision}\n0\n0\n")
-    out_d, tail_d = _run_host(EXE, tmp_path / "device", control, f"{precision}\n0\n1\n")
+    print(f"fortran host loops precision={precision} engine={engine} dt0={dt0}", flush=True)
+    out_h, tail_h = _run_host(EXE, tmp_path / "host", control, f"{precision}\n{engine}\n0\n")
+    out_d, tail_d = _run_host(EXE, tmp_path / "device", control, f"{precision}\n{engine}\n1\n")
     assert out_d == out_h
     assert tail_d.split("wall:")[0] == tail_h.split("wall:")[0]
     if dt0 > 1e-3:
