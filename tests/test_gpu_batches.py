@@ -181,3 +181,60 @@ def test_outflow_mass_budget(prec, engine):
         # eps q / (2 s), about 1e-11; the FP32 engine drops those terms)
         assert np.abs(quiet1 - quiet0).max() <= 1e-9
     _say(f"  worst |dM - budget| = {worst:.3e}")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_runs_stop_at_snapshots(prec):
+    """pcw_stop_at_snapshots on a batch of three instances with their own dt:
+    each device-controlled run stops every instance after the accepted step
+    that reaches its next snapshot time (the instance's time is that snapshot
+    time exactly, the last record of its log marks the snapshot, no earlier
+    record does), and the next run continues it.  Runs stopped at every
+    snapshot end in the states of one uninterrupted run, bit for bit, with the
+    same accepted and rejected counts (the time loop of src/main.f90:193-283
+    returning to the host for snapshot output at :256-268)."""
+    from paraclaws_amd import abi, ic
+
+    members = [(64, 2.0), (96, 4.0), (130, 1.5)]
+    precision = abi.FP64 if prec == "fp64" else abi.FP32
+    t_final, n_snap = 0.05, 5
+    kw = dict(TIME_KW, t_final=t_final, n_snapshots=n_snap)
+    interval = t_final / n_snap
+    snaps = [i * interval for i in range(1, n_snap + 1)]
+    snaps[-1] = t_final
+
+    def engine():
+        e = abi.Engine([abi.Domain(n, n, bc=(4, 4, 4, 4)) for n, _ in members], precision=precision,
+                       limiter=4)
+        for k, (n, r) in enumerate(members):
+            e.set_state(k, ic.dam_break(n, n, h_in=r))
+        e.fill_ghosts()
+        e.time_init(**kw)
+        return e
+
+    _say(f"stop at snapshots {prec}: uninterrupted run")
+    with engine() as ref:
+        ref.run(steps_per_graph=16)
+        want = [(ref.get_state(k), ref.status(k)) for k in range(len(members))]
+    _say(f"stop at snapshots {prec}: runs stopped at every snapshot")
+    with engine() as e:
+        e.stop_at_snapshots(True)
+        reached = {k: [] for k in range(len(members))}
+        for _ in range(n_snap):
+            e.enable_log(100000)
+            e.run(steps_per_graph=16)
+            for k in range(len(members)):
+                log = e.read_log(k)
+                assert log, (k, reached[k])
+                assert all(r[1] == 0 for r in log[:-1])
+                acc, snap, *_ = log[-1]
+                assert acc == 1 and snap == len(reached[k]) + 1
+                assert e.status(k).time == snaps[snap - 1]
+                reached[k].append(snap)
+        for k in range(len(members)):
+            st = e.status(k)
+            assert reached[k] == list(range(1, n_snap + 1))
+            assert st.status == abi.STATUS_DONE and st.time == t_final
+            q, st_ref = want[k]
+            assert (st.accepted, st.rejected) == (st_ref.accepted, st_ref.rejected)
+            assert np.array_equal(e.get_state(k).view(np.uint64), q.view(np.uint64))
