@@ -264,7 +264,46 @@ def cpu_baseline(n, target_s=15.0):
     acc = sum(r[0] for r in recs)
     return dict(value=acc * n * n / el, unit="cell-updates/s", cores=cores, kind="port",
                 sample=f"dam break {n}x{n}, first {acc} accepted steps from the initial state, "
-                       f"FP64 CPU restatement with OpenMP ({el:.1f} s)")
+                       f"FP64 CPU restatement with OpenMP ({el:.1f} s)",
+                reference_structured=cpu_reference_structured(cores))
+
+
+def cpu_reference_structured(cores, n=256, steps=10):
+    """The reference's CPU cost structure: the restatement with LAPACK dgeev
+    (workspace query, right eigenvectors) for every cell of every interface and
+    BLAS dgemv for J*q, as SMOD_approximate_Riemann_solvers.f90:60-101 calls
+    them (scipy's LAPACK, one BLAS thread as MKL_NUM_THREADS=1, OpenMP over the
+    same threads), on the first `steps` accepted steps of config 1 (256^2);
+    beside it the closed-form restatement on the same steps and the largest
+    relative difference of their states."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import oracle as O
+
+    cfg = O.Config(nx=n, ny=n, limiter=4, bc=(4, 4, 4, 4), nthreads=cores)
+    ts = O.TimeSettings(t_final=0.5, dt0=1e-10, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10,
+                        growth=1.2, shrink=0.5, n_snap=10)
+    q_ref = O.dam_break(cfg)
+    with threadpool_limits(1):
+        O.use_lapack(True)
+        try:
+            t0 = time.perf_counter()
+            recs, _ = O.run(cfg, ts, q_ref, max_accepted=steps)
+            el = time.perf_counter() - t0
+        finally:
+            O.use_lapack(False)
+    q_cf = O.dam_break(cfg)
+    t0 = time.perf_counter()
+    O.run(cfg, ts, q_cf, max_accepted=steps)
+    el_cf = time.perf_counter() - t0
+    acc = sum(r[0] for r in recs)
+    inner = (slice(2, -2), slice(2, -2))
+    rel = float((np.abs(q_ref[inner] - q_cf[inner]).max(axis=(0, 1))
+                 / np.abs(q_cf[inner]).max(axis=(0, 1))).max())
+    return dict(value=acc * n * n / el, unit="cell-updates/s", cores=cores, kind="port",
+                sample=f"dam break {n}x{n}, first {acc} accepted steps, per-interface LAPACK dgeev "
+                       f"+ BLAS dgemv ({el:.2f} s)",
+                closed_form_same_steps=acc * n * n / el_cf, max_rel_diff_vs_closed_form=rel)
 
 
 def cpu_config1(abi, ic):

@@ -106,8 +106,37 @@ def lib():
         L.orc_phi.restype = C.c_double
         L.orc_totals.argtypes = [P(OrcCfg), dp, dp]
         L.orc_totals.restype = None
+        L.orc_use_lapack.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_use_lapack.restype = None
         _lib = L
     return _lib
+
+
+def use_lapack(on: bool = True) -> None:
+    """Switch the restatement to the reference's per-interface cost structure
+    (SMOD_approximate_Riemann_solvers.f90:60-101: LAPACK dgeev for the
+    eigenvalues of each cell's Jacobian, with its workspace query and right
+    eigenvectors, and BLAS dgemv for J*q), with scipy's LAPACK and BLAS; off:
+    the closed forms.  Process-wide.  Callers limit the BLAS threads to one,
+    as the reference's MKL_NUM_THREADS=1 (set_env.sh:24)."""
+    L = lib()
+    if not on:
+        L.orc_use_lapack(None, None)
+        return
+    from scipy.linalg import cython_blas, cython_lapack
+
+    name = C.pythonapi.PyCapsule_GetName
+    name.restype = C.c_char_p
+    name.argtypes = [C.py_object]
+    ptr = C.pythonapi.PyCapsule_GetPointer
+    ptr.restype = C.c_void_p
+    ptr.argtypes = [C.py_object, C.c_char_p]
+
+    def fn(mod, sym):
+        cap = mod.__pyx_capi__[sym]
+        return ptr(cap, name(cap))
+
+    L.orc_use_lapack(fn(cython_lapack, "dgeev"), fn(cython_blas, "dgemv"))
 
 
 def _dp(a):

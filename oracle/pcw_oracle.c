@@ -288,6 +288,72 @@ static inline int phys_y(const orc_cfg* c, const double q[3], double* s,
   return 0;
 }
 
+/* ----------------------------- the reference's per-interface structure
+ * Optional (orc_use_lapack): each cell of an interface gets the reference's
+ * Jacobian (examples/2D_dam_break/cartesian/SMOD_jakobian.f90:44-53, column
+ * major), its eigenvalues from LAPACK dgeev exactly as calculateEigenSolution
+ * calls it (src/utility/SMOD_eigenvalue_problem.f90:5-89: a workspace query,
+ * an allocated work array, right eigenvectors requested, complex check) and
+ * J*q from BLAS dgemv (SMOD_matrix_operations.f90:5-23).  This is the cost
+ * structure of the reference's CPU path; its values agree with the closed
+ * forms above to rounding (the eigenvalues are LAPACK's, J*q is BLAS's). */
+typedef void (*orc_dgeev_fn)(char*, char*, int*, double*, int*, double*, double*, double*, int*,
+                             double*, int*, double*, int*, int*);
+typedef void (*orc_dgemv_fn)(char*, int*, int*, double*, double*, int*, double*, int*, double*,
+                             double*, int*);
+static orc_dgeev_fn g_dgeev = 0;
+static orc_dgemv_fn g_dgemv = 0;
+
+void orc_use_lapack(void* dgeev, void* dgemv) {
+  g_dgeev = (dgeev && dgemv) ? (orc_dgeev_fn)dgeev : 0;
+  g_dgemv = (dgeev && dgemv) ? (orc_dgemv_fn)dgemv : 0;
+}
+
+static int phys_lapack(const orc_cfg* c, int ixy, const double q[3], double* s, double fl[3]) {
+  const double g = c->grav, h = q[0], hu = q[1], hv = q[2];
+  double J[9] = {0.0};  /* J(k, l) at J[(l - 1) * 3 + (k - 1)] */
+#define JM(k, l) J[((l) - 1) * 3 + ((k) - 1)]
+  if (ixy == 1) {
+    JM(1, 2) = 1.0;
+    JM(2, 1) = -(hu * hu / (h * h)) + g * h;
+    JM(2, 2) = 2.0 * hu / h;
+    JM(3, 1) = -(hu * hv / (h * h));
+    JM(3, 2) = hv / h;
+    JM(3, 3) = hu / h;
+  } else {
+    JM(1, 3) = 1.0;
+    JM(2, 1) = -(hu * hv / (h * h));
+    JM(2, 2) = hv / h;
+    JM(2, 3) = hu / h;
+    JM(3, 1) = -(hv * hv / (h * h)) + g * h;
+    JM(3, 3) = 2.0 * hv / h;
+  }
+#undef JM
+  for (int k = 0; k < 9; ++k)
+    if (isnan(J[k])) return 1;
+  char jobvl = 'N', jobvr = 'V';
+  int n = 3, lda = 3, ldvl = 1, ldvr = 3, lwork = -1, info = 0;
+  double A[9], wr[3] = {0.0}, wi[3] = {0.0}, vl[1] = {0.0}, R[9] = {0.0}, worktmp[1] = {0.0};
+  memcpy(A, J, sizeof A);
+  double* work = malloc(sizeof(double));
+  g_dgeev(&jobvl, &jobvr, &n, A, &lda, wr, wi, vl, &ldvl, R, &ldvr, worktmp, &lwork, &info);
+  lwork = (int)worktmp[0];
+  free(work);
+  work = calloc((size_t)(lwork > 1 ? lwork : 1), sizeof(double));
+  g_dgeev(&jobvl, &jobvr, &n, A, &lda, wr, wi, vl, &ldvl, R, &ldvr, work, &lwork, &info);
+  free(work);
+  for (int k = 0; k < 3; ++k)
+    if (wi[k] > c->eps) return 2;
+  double sm = 0.0;
+  for (int k = 0; k < 3; ++k) sm = fabs(wr[k]) > sm ? fabs(wr[k]) : sm;
+  *s = sm;
+  char tr = 'N';
+  int inc = 1;
+  double one = 1.0, zero = 0.0, x[3] = {q[0], q[1], q[2]};
+  g_dgemv(&tr, &n, &n, &one, J, &lda, x, &inc, &zero, fl, &inc);
+  return 0;
+}
+
 /* --------------------------------------------------------- work arrays
  * src/solver/MOD_work_arrays.f90:14-70; zeroed after every sweep (:78-109). */
 typedef struct {
@@ -333,8 +399,14 @@ static void work_zero(work_t* w) {
 static inline int hll(const orc_cfg* c, int ixy, const double* qL,
                       const double* qR, work_t* w, long k) {
   double sL, sR, fL[3], fR[3];
-  int e1 = (ixy == 1) ? phys_x(c, qL, &sL, fL) : phys_y(c, qL, &sL, fL);
-  int e2 = (ixy == 1) ? phys_x(c, qR, &sR, fR) : phys_y(c, qR, &sR, fR);
+  int e1, e2;
+  if (g_dgeev && !(c->dry_tol > 0.0)) {  /* the reference's structure (no wet/dry in it) */
+    e1 = phys_lapack(c, ixy, qL, &sL, fL);
+    e2 = phys_lapack(c, ixy, qR, &sR, fR);
+  } else {
+    e1 = (ixy == 1) ? phys_x(c, qL, &sL, fL) : phys_y(c, qL, &sL, fL);
+    e2 = (ixy == 1) ? phys_x(c, qR, &sR, fR) : phys_y(c, qR, &sR, fR);
+  }
   if (e1 || e2) return 1;
   double s2 = sL > sR ? sL : sR;
   double s1 = -s2;

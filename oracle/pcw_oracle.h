@@ -135,6 +135,12 @@ double orc_phi(int code, double r);
  * by kappa*dxi*deta). */
 void orc_totals(const orc_cfg* c, const double* vars, double out[3]);
 
+/* The reference's per-interface cost structure (LAPACK dgeev for the
+ * eigenvalues, BLAS dgemv for J*q, per cell of every interface) instead of the
+ * closed forms: the Fortran-convention dgeev and dgemv of a LAPACK/BLAS the
+ * caller supplies; NULL switches back to the closed forms.  Process-wide. */
+void orc_use_lapack(void* dgeev, void* dgemv);
+
 #ifdef __cplusplus
 }
 #endif
