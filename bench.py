@@ -122,19 +122,21 @@ def replay_fp32(abi, ic, n, dts):
     return e
 
 
-def run_ensemble(abi, ic, spg):
+def run_ensemble(abi, ic, spg, tiles=None):
     """512 dam breaks of uneven sizes (64^2..1024^2, seed 4), reflective walls,
-    own adaptive dt each, one step launch per graph node, FP32, to t = 0.1."""
+    own adaptive dt each, one step launch per graph node, FP32, to t = 0.1.
+    tiles: (rows, waves per block) instead of the autotuner's."""
     members = ic.ensemble_config4()
     doms = [abi.Domain(n, n, bc=(4, 4, 4, 4)) for n, _ in members]
-    e = abi.Engine(doms, precision=abi.FP32, limiter=4)
+    e = abi.Engine(doms, precision=abi.FP32, limiter=4,
+                   **(dict(tile_rows=tiles[0], waves_per_block=tiles[1]) if tiles else {}))
     for k, (n, r) in enumerate(members):
         e.set_state(k, ic.dam_break(n, n, h_in=r))
     e.fill_ghosts()
     tkw = dict(TIME_KW)
     tkw["t_final"] = 0.1
     e.time_init(**tkw)
-    tuned = e.autotune(steps=2)
+    tuned = tuple(tiles) if tiles else e.autotune(steps=2)
     e.time_init(**tkw)
     sync()
     t0 = time.perf_counter()
@@ -195,10 +197,11 @@ def run_bathymetry(abi, ic, n, steps, spg, tiles=None, warmup=10):
                 tuned=tuned)
 
 
-def run_friction_batch(abi, ic, spg, t_final=2.0, n_inst=64):
+def run_friction_batch(abi, ic, spg, t_final=2.0, n_inst=64, tiles=None):
     """Config 5: 64 instances of 2048 x 1024 on [-2,2]x[-1,1], Manning n
     uniform in [0.01, 0.05] and Gaussian-sum bathymetry per instance (seed 5),
-    superbee, outflow in x, walls in y, FP32 state, FP64 totals, to t = 2.0."""
+    superbee, outflow in x, walls in y, FP32 state, FP64 totals, to t = 2.0.
+    tiles: (rows, waves per block) instead of the autotuner's."""
     g = ic.SplitMix64(5)
     nx, ny = 2048, 1024
     doms, bs, ns = [], [], []
@@ -207,7 +210,8 @@ def run_friction_batch(abi, ic, spg, t_final=2.0, n_inst=64):
         ns.append(nman)
         doms.append(abi.Domain(nx, ny, bc=(2, 2, 4, 4), xi=(-2.0, 2.0), eta=(-1.0, 1.0),
                                source=abi.SRC_BATHY_FRICTION, manning_n=nman))
-    e = abi.Engine(doms, precision=abi.FP32, limiter=2, dry_tol=1e-6)
+    e = abi.Engine(doms, precision=abi.FP32, limiter=2, dry_tol=1e-6,
+                   **(dict(tile_rows=tiles[0], waves_per_block=tiles[1]) if tiles else {}))
     m0 = 0.0
     for k in range(n_inst):
         b = ic.gaussian_bathymetry(nx, ny, seed=5000 + k, xi=(-2.0, 2.0), eta=(-1.0, 1.0))
@@ -220,7 +224,7 @@ def run_friction_batch(abi, ic, spg, t_final=2.0, n_inst=64):
     tkw = dict(TIME_KW)
     tkw["t_final"] = t_final
     e.time_init(**tkw)
-    tuned = e.autotune(steps=2)
+    tuned = tuple(tiles) if tiles else e.autotune(steps=2)
     e.time_init(**tkw)
     sync()
     t0 = time.perf_counter()
@@ -481,6 +485,11 @@ def main():
                     help="bathymetry + wet/dry (config 3) run only, for rocprofv3 traces and counters")
     ap.add_argument("--profile-fp64", action="store_true",
                     help="FP64 fused (bit-exact) run only, for rocprofv3 kernel traces")
+    ap.add_argument("--profile-ensemble", action="store_true",
+                    help="the 512-instance ensemble (config 4) only, for rocprofv3 kernel traces")
+    ap.add_argument("--profile-friction", type=float, default=0.0, metavar="T",
+                    help="the friction batch (config 5) to simulated time T only, for rocprofv3 "
+                         "kernel traces")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB (FETCH/WRITE counter scale)")
     args = ap.parse_args()
@@ -512,6 +521,16 @@ def main():
                              warmup=args.warmup)
         print(json.dumps({k: (float(v) if hasattr(v, "dtype") else v) for k, v in bat.items()}),
               flush=True)
+        return
+    if args.profile_ensemble:
+        log("batched ensemble: 512 dam breaks 64^2..1024^2 (seed 4), FP32, to t = 0.1")
+        print(json.dumps(run_ensemble(abi, ic, 64, tiles=args.tiles)), flush=True)
+        return
+    if args.profile_friction > 0:
+        log(f"friction batch: 64 x 2048x1024, Manning + bathymetry, superbee, to t = "
+            f"{args.profile_friction}")
+        print(json.dumps(run_friction_batch(abi, ic, 64, t_final=args.profile_friction,
+                                            tiles=args.tiles)), flush=True)
         return
     if args.profile_fp64:
         log(f"FP64 fused engine, dam break {n}^2, {args.steps} accepted steps")
