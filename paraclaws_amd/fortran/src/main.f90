@@ -12,7 +12,8 @@ program paraCLAWs_gpu
     use MOD_IO
     use MOD_work_arrays, only: work_arrays
     use MOD_device, only: device_upload, device_download, device_commit, device_totals, &
-        device_source_kind, device_loop_start, device_loop_segment, device_time, segment_attempts
+        device_source_kind, device_loop_start, device_loop_segment, device_time, segment_attempts, &
+        device_autotune
     use MOD_pcw, only: pcw_step_record
     use MOD_solver_parameters, only: device_loop
     use MOD_error_handling, only: abort_program
@@ -81,6 +82,7 @@ program paraCLAWs_gpu
     call update_rank_boundaries(decomposition, mesh, solution)
     call device_totals(totals)
     write(*,3) totals
+    call device_autotune()
 
     n_accepted = 0
     n_rejected = 0

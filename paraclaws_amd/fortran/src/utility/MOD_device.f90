@@ -122,6 +122,17 @@ contains
         ghosts_current = .true.
     end subroutine
 
+    subroutine device_autotune()
+        !! Tile shape of the step kernels for this domain (pcw_autotune: a few
+        !! timed attempts per candidate that are never kept; the state and the
+        !! run are unchanged).  Only the fused engine has tiles.
+        use MOD_solver_parameters, only: device_engine
+        integer(c_int32_t) :: rows, waves
+        if (device_engine /= PCW_ENGINE_FUSED) return
+        call device_check(pcw_autotune(handle, 3_c_int32_t, rows, waves), "pcw_autotune")
+        write(*, '(a,i0,a,i0,a)') ' device tiles: ', rows, ' rows, ', waves, ' waves per block'
+    end subroutine
+
     subroutine device_loop_start(t_ctrl)
         !! The time loop on the device: a copy of the controller (pcw_time_init
         !! with control.in's settings, at the initial time) that stops the run

@@ -177,6 +177,15 @@ module MOD_pcw
             integer(c_int) :: rc
         end function
 
+        function pcw_autotune(h, steps, tile_rows, waves_per_block) bind(C, name="pcw_autotune") &
+                result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: steps
+            integer(c_int32_t), intent(out) :: tile_rows, waves_per_block
+            integer(c_int) :: rc
+        end function
+
         function pcw_get_status(h, inst, out) bind(C, name="pcw_get_status") result(rc)
             import :: c_int, c_int32_t, c_ptr, pcw_status
             type(c_ptr), value :: h

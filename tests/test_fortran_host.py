@@ -407,7 +407,9 @@ def _run_host(exe, case_dir, control, custom):
     (case_dir / "custom.in").write_text(custom)
     r = subprocess.run([exe], cwd=case_dir, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
-    lines = r.stdout.splitlines()
+    # the tile shape the program picks (timed, so it may differ between runs)
+    # does not change the run
+    lines = [ln for ln in r.stdout.splitlines() if not ln.startswith(" device tiles:")]
     first = next(k for k, ln in enumerate(lines) if "paraCLAWs - time:" in ln)
     last = next(k for k, ln in enumerate(lines) if ln.startswith("accepted steps:"))
     return lines[first:last], lines[last]
