@@ -125,3 +125,57 @@ def test_fp32_config1_to_final_time_within_the_perturbation_floor(cfl):
     print(f"  fp64 from a 2^-24 perturbed start={floor}", flush=True)
     assert np.isfinite(got).all()
     assert err["h"]["l1_rel"] <= floor["h"]["l1_rel"]
+
+
+@pytest.mark.parametrize("cfl", [0.45, 0.9])
+def test_fp32_bathymetry_against_fp64_and_its_rounding_floor(cfl):
+    """Config 3's physics (Gaussian-sum bathymetry of seed 3, still water with
+    dry cells, dam-break step on x < 0, outflow sides, wet/dry threshold 1e-6,
+    MC) at 2048^2 for 200 accepted steps: the FP32 engine replaying the FP64
+    fused engine's dt sequence, against the FP64 state, and the floor the
+    scheme itself sets -- the FP64 engine replaying the same dt sequence from
+    the inputs rounded once to FP32 (state and bathymetry).
+    * CFL 0.45: h relative L1 within the 1e-5 tolerance (5.4e-7 measured;
+      floor 2.9e-8).
+    * CFL 0.9: the reference's update amplifies perturbations there, and the
+      bathymetry excites that everywhere, so one rounding of the inputs
+      already grows to 2.2e-4 in h; FP32 (a rounding every step) stays within
+      a small factor of that floor (1.6e-3, 7.4 times, measured; bound 20)."""
+    from paraclaws_amd import abi, ic
+
+    n, steps = 2048, 200
+    b = ic.gaussian_bathymetry(n, n, seed=3)
+    q = ic.still_water_with_step(b, n, dry_tol=1e-6)
+    dom = abi.Domain(n, n, bc=(2, 2, 2, 2), source=abi.SRC_BATHY)
+    kw = dict(TIME_KW, cfl_target=cfl)
+    print(f"bathymetry accuracy {n}^2 CFL {cfl}", flush=True)
+    with abi.Engine([dom], precision=abi.FP64, limiter=4, dry_tol=1e-6) as e:
+        e.set_state(0, q)
+        e.set_bathymetry(0, b)
+        e.fill_ghosts()
+        e.time_init(**kw)
+        e.enable_log(4 * steps + 64)
+        e.run(steps_per_graph=32, max_accepted=steps)
+        dts = [r[2] for r in e.read_log(0) if r[0] == 1]
+        ref = e.get_state(0)
+    assert len(dts) == steps
+
+    def replay(prec, qq, bb):
+        with abi.Engine([dom], precision=prec, limiter=4, dry_tol=1e-6) as e:
+            e.set_state(0, qq)
+            e.set_bathymetry(0, bb)
+            e.fill_ghosts()
+            for dt in dts:
+                e.step(dt)
+                e.accept()
+            return e.get_state(0)
+
+    err = ic.field_errors(replay(abi.FP32, q, b), ref)
+    floor = ic.field_errors(replay(abi.FP64, q.astype(np.float32).astype(np.float64),
+                                   b.astype(np.float32).astype(np.float64)), ref)
+    print(f"  fp32 {err}\n  fp64 from fp32-rounded inputs {floor}", flush=True)
+    if cfl < 0.5:
+        assert err["h"]["l1_rel"] <= H_L1_TOL
+    else:
+        assert err["h"]["l1_rel"] <= 20 * floor["h"]["l1_rel"]
+    assert floor["h"]["l1_rel"] < err["h"]["l1_rel"]
