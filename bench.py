@@ -58,6 +58,7 @@ def sync():
         pass
 
 
+T_ENS_BENCH = 0.1
 TIME_KW = dict(t_final=0.5, dt0=1e-10, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10, growth=1.2,
                shrink=0.5, n_snapshots=10)
 
@@ -134,7 +135,7 @@ def run_ensemble(abi, ic, spg, tiles=None):
         e.set_state(k, ic.dam_break(n, n, h_in=r))
     e.fill_ghosts()
     tkw = dict(TIME_KW)
-    tkw["t_final"] = 0.1
+    tkw["t_final"] = T_ENS_BENCH
     e.time_init(**tkw)
     tuned = tuple(tiles) if tiles else e.autotune(steps=2)
     e.time_init(**tkw)
@@ -197,45 +198,102 @@ def run_bathymetry(abi, ic, n, steps, spg, tiles=None, warmup=10):
                 tuned=tuned)
 
 
-def bathymetry_errors(abi, ic, n, steps, spg):
-    """Config 3's FP32 error: the FP64 fused engine (the reference's arithmetic
-    order, with this build's bathymetry slope and wet/dry cut-off) on the same
-    inputs for `steps` accepted steps under the device controller, and the FP32
-    engine replaying its accepted dt sequence; per-field relative L1/Linf.
-    Beside it the floor the scheme itself sets: the FP64 engine replaying the
-    same dt sequence from the inputs rounded once to FP32 (state and
-    bathymetry).  At CFL 0.9 the reference's update amplifies perturbations,
-    and the bathymetry excites that everywhere, so a single rounding of the
-    inputs already grows far above FP32 resolution."""
-    b = ic.gaussian_bathymetry(n, n, seed=3)
-    q = ic.still_water_with_step(b, n, dry_tol=1e-6)
-    dom = abi.Domain(n, n, bc=(2, 2, 2, 2), source=abi.SRC_BATHY)
-    with abi.Engine([dom], precision=abi.FP64, limiter=4, dry_tol=1e-6) as e:
+def fp32_errors_vs_fp64(abi, ic, dom, q, time_kw, engine_kw, bathy=None, max_accepted=0, spg=32):
+    """One domain's FP32 error: the FP64 fused engine (the reference's
+    arithmetic order) runs under its device controller, logging every attempt;
+    the FP32 engine replays the accepted dt sequence from the same inputs.  Two
+    floors the scheme itself sets, both FP64 replays of the same dt sequence:
+    from the inputs rounded once to FP32 (state and bathymetry), and from a
+    start whose depth carries a relative perturbation of 2^-24 (one FP32
+    rounding) with a random sign per cell, which excites every mode even
+    where the inputs are exact in FP32 (a dam break's two depths).  Per-field
+    relative L1/Linf against the FP64 state.  At CFL 0.9 the reference's
+    update amplifies perturbations (above CFL 0.5 it is not von Neumann
+    stable), so one rounding can grow far above FP32 resolution; a perturbed
+    replay that the engine rejects (the perturbation grew into a negative
+    depth) is reported with the engine's message."""
+    with abi.Engine([dom], precision=abi.FP64, **engine_kw) as e:
         e.set_state(0, q)
-        e.set_bathymetry(0, b)
+        if bathy is not None:
+            e.set_bathymetry(0, bathy)
         e.fill_ghosts()
-        e.time_init(**TIME_KW)
-        e.enable_log(4 * steps + 64)
-        e.run(steps_per_graph=spg, max_accepted=steps)
+        e.time_init(**time_kw)
+        e.enable_log(200000)
+        e.run(steps_per_graph=spg, max_accepted=max_accepted)
         dts = [r[2] for r in e.read_log(0) if r[0] == 1]
-        t = e.status(0).time
+        st = e.status(0)
+        assert len(dts) == st.accepted
         ref = e.get_state(0)
+
     def replay(prec, qq, bb):
-        with abi.Engine([dom], precision=prec, limiter=4, dry_tol=1e-6) as e:
+        with abi.Engine([dom], precision=prec, **engine_kw) as e:
             e.set_state(0, qq)
-            e.set_bathymetry(0, bb)
+            if bb is not None:
+                e.set_bathymetry(0, bb)
             e.fill_ghosts()
             for dt in dts:
                 e.step(dt)
                 e.accept()
             return e.get_state(0)
 
-    errs = ic.field_errors(replay(abi.FP32, q, b), ref)
-    floor = ic.field_errors(replay(abi.FP64, q.astype(np.float32).astype(np.float64),
-                                   b.astype(np.float32).astype(np.float64)), ref)
-    return dict(errors=errs, fp64_from_fp32_rounded_inputs=floor, accepted_steps=len(dts), time=t,
+    r32 = lambda a: None if a is None else a.astype(np.float32).astype(np.float64)
+    qp = q.copy()
+    rng = np.random.default_rng(1)
+    qp[..., 0] *= 1.0 + rng.choice([-1.0, 1.0], size=qp.shape[:2]) * 2.0 ** -24
+    try:
+        perturbed = ic.field_errors(replay(abi.FP64, qp, bathy), ref)
+    except abi.PcwError as err:
+        perturbed = f"FP64 replay stopped: {err}"
+    return dict(errors=ic.field_errors(replay(abi.FP32, q, bathy), ref),
+                fp64_from_fp32_rounded_inputs=ic.field_errors(replay(abi.FP64, r32(q), r32(bathy)), ref),
+                fp64_from_2pow24_perturbed_depth=perturbed,
+                accepted_steps=len(dts), time=st.time,
                 reference="FP64 fused engine on the same inputs",
                 fp32="replay of the reference's accepted dt sequence")
+
+
+def bathymetry_errors(abi, ic, n, steps, spg):
+    """Config 3's FP32 error and its floor (fp32_errors_vs_fp64) over `steps`
+    accepted steps."""
+    b = ic.gaussian_bathymetry(n, n, seed=3)
+    q = ic.still_water_with_step(b, n, dry_tol=1e-6)
+    dom = abi.Domain(n, n, bc=(2, 2, 2, 2), source=abi.SRC_BATHY)
+    return fp32_errors_vs_fp64(abi, ic, dom, q, TIME_KW, dict(limiter=4, dry_tol=1e-6), bathy=b,
+                               max_accepted=steps, spg=spg)
+
+
+def ensemble_member_errors(abi, ic):
+    """Config 4's FP32 error on the smallest, the median and the largest
+    member run alone to t = 0.1 (a member alone equals the batch bit for bit),
+    at the ensemble's CFL 0.9, and the largest member again at CFL 0.45, where
+    the update is stable."""
+    members = ic.ensemble_config4()
+    order = sorted(range(len(members)), key=lambda k: members[k][0])
+    out = []
+    for k, cfl in ((order[0], 0.9), (order[len(order) // 2], 0.9), (order[-1], 0.9), (order[-1], 0.45)):
+        n, r = members[k]
+        rec = fp32_errors_vs_fp64(abi, ic, abi.Domain(n, n, bc=(4, 4, 4, 4)),
+                                  ic.dam_break(n, n, h_in=r),
+                                  dict(TIME_KW, t_final=T_ENS_BENCH, cfl_target=cfl), dict(limiter=4))
+        rec.update(member=k, n=n, depth_ratio=r, cfl_target=cfl)
+        out.append(rec)
+    return out
+
+
+def friction_member_errors(abi, ic, t_final=2.0):
+    """Config 5's FP32 error on its first member (2048 x 1024, Manning n and
+    bathymetry of seed 5000) run alone to t_final."""
+    g = ic.SplitMix64(5)
+    nman = g.uniform(0.01, 0.05)
+    nx, ny = 2048, 1024
+    dom = abi.Domain(nx, ny, bc=(2, 2, 4, 4), xi=(-2.0, 2.0), eta=(-1.0, 1.0),
+                     source=abi.SRC_BATHY_FRICTION, manning_n=nman)
+    b = ic.gaussian_bathymetry(nx, ny, seed=5000, xi=(-2.0, 2.0), eta=(-1.0, 1.0))
+    q = ic.still_water_with_step(b, nx, xi=(-2.0, 2.0), dry_tol=1e-6)
+    rec = fp32_errors_vs_fp64(abi, ic, dom, q, dict(TIME_KW, t_final=t_final),
+                              dict(limiter=2, dry_tol=1e-6), bathy=b)
+    rec.update(member=0, manning_n=nman)
+    return rec
 
 
 def run_friction_batch(abi, ic, spg, t_final=2.0, n_inst=64, tiles=None):
@@ -697,6 +755,9 @@ def main():
         log("batched ensemble: 512 dam breaks 64^2..1024^2 (seed 4), FP32, to t = 0.1")
         ens = run_ensemble(abi, ic, 64)
         log(f"  {ens}")
+        log("ensemble: FP32 error of the smallest, median and largest member against FP64")
+        ens["errors_fp32_vs_fp64_members"] = ensemble_member_errors(abi, ic)
+        log(f"  {ens['errors_fp32_vs_fp64_members']}")
         out_extra["batched_ensemble"] = ens
 
     if not args.no_physics:
@@ -709,6 +770,8 @@ def main():
         out_extra["bathymetry"] = bat
         log("friction batch: 64 x 2048x1024, Manning + bathymetry, superbee, to t = 2.0")
         fr = run_friction_batch(abi, ic, 64)
+        log("friction batch: FP32 error of the first member against FP64, to t = 2.0")
+        fr["errors_fp32_vs_fp64_member0"] = friction_member_errors(abi, ic)
         log(f"  {fr}")
         out_extra["friction_batch"] = fr
         log(f"stretched grid (x = 4 xi, y = 2 eta), metric arrays, FP32 {n}^2, {args.steps} steps")
