@@ -68,6 +68,11 @@ extern "C" {
 #define PCW_STATUS_DONE 1
 #define PCW_STATUS_ERROR 2
 
+/* Size limits, checked by pcw_create and pcw_step_noalloc (PCW_ERR_ARG):
+ * every kernel's grid then stays within 65,535 workgroups in y and z. */
+#define PCW_MAX_SIDE 262136     /* nx, ny of one domain */
+#define PCW_MAX_DOMAINS 65535   /* instances of one handle */
+
 typedef struct pcw_handle pcw_handle;
 
 /* Per-instance numerical settings; field meaning follows control.in
@@ -118,7 +123,12 @@ typedef struct pcw_status {
   double time, dt, cfl_last;
 } pcw_status;
 
-/* ---- lifetime ---------------------------------------------------------- */
+/* ---- lifetime ----------------------------------------------------------
+ * pcw_create: PCW_ERR_ARG, with a message, for sizes beyond the limits
+ * above, nx or ny below 2 and bad settings; a handle is returned then too,
+ * for pcw_last_error, and is released with pcw_destroy.  FP32 tiles span at
+ * most the rows whose three planes' 32-bit buffer offsets stay below 2^31
+ * bytes per plane: a larger tile_rows is lowered to that for the instance. */
 int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains,
                pcw_handle** out);
 int pcw_destroy(pcw_handle* h);

@@ -22,6 +22,7 @@ FP64, FP32 = 0, 1
 FUSED, STAGED = 0, 1
 BC_ZERO_GRADIENT, BC_CONSTANT_GRADIENT, BC_REFLECTIVE = 2, 3, 4
 SRC_NONE, SRC_BATHY, SRC_FRICTION, SRC_BATHY_FRICTION = 0, 1, 2, 3
+PCW_MAX_SIDE, PCW_MAX_DOMAINS = 262136, 65535   # size limits of pcw_create
 STATUS_RUNNING, STATUS_DONE, STATUS_ERROR = 0, 1, 2
 
 # g = 9.81 as a default-kind (REAL*4) literal widened to double, as the

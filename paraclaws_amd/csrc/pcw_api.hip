@@ -244,8 +244,8 @@ long long count_tiles(const pcw_handle* h, int ty, const std::vector<char>* only
     if (only && !(*only)[k]) continue;
     const KInst& I = h->hinst[k];
     const int nsx = (I.nx + kStripOut - 1) / kStripOut;
-    row_bands(I.ny, ty, shorten, false, inner);
-    row_bands(I.ny, ty, shorten, shorten, edge);
+    row_bands(I.ny, tile_rows_for(I, ty), shorten, false, inner);
+    row_bands(I.ny, tile_rows_for(I, ty), shorten, shorten, edge);
     const int nedge = std::min(nsx, 2);
     t += (long long)(nsx - nedge) * (long long)inner.size() + (long long)nedge * (long long)edge.size();
   }
@@ -271,8 +271,8 @@ void build_tiles(pcw_handle* h, int ty, const std::vector<char>* only = nullptr)
       if (x0 + kStripOut - 1 > I.nx) x0 = std::max(1, I.nx - kStripOut + 1);
       xs.push_back(x0);
     }
-    row_bands(I.ny, ty, shorten, false, inner);
-    row_bands(I.ny, ty, shorten, shorten, edge);
+    row_bands(I.ny, tile_rows_for(I, ty), shorten, false, inner);
+    row_bands(I.ny, tile_rows_for(I, ty), shorten, shorten, edge);
     std::vector<int4> t;
     for (int s = 0; s < nsx; ++s) {
       const bool is_edge = s == 0 || s == nsx - 1;
@@ -573,6 +573,11 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
   h->p = *p;
   h->n = n_domains;
   h->esize = (p->precision == PCW_FP32) ? 4 : 8;
+  if (n_domains > PCW_MAX_DOMAINS) {
+    fail(h, PCW_ERR_ARG, "a handle holds at most PCW_MAX_DOMAINS (65,535) domains");
+    *out = h;
+    return PCW_ERR_ARG;
+  }
   if (p->precision != PCW_FP32 && p->precision != PCW_FP64) {
     fail(h, PCW_ERR_ARG, "precision must be PCW_FP64 or PCW_FP32");
     *out = h;
@@ -597,6 +602,11 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
     const pcw_domain& d = domains[k];
     if (d.nx < 2 || d.ny < 2) {
       fail(h, PCW_ERR_ARG, "each domain needs nx >= 2 and ny >= 2");
+      *out = h;
+      return PCW_ERR_ARG;
+    }
+    if (d.nx > PCW_MAX_SIDE || d.ny > PCW_MAX_SIDE) {
+      fail(h, PCW_ERR_ARG, "each domain needs nx, ny <= PCW_MAX_SIDE (262,136)");
       *out = h;
       return PCW_ERR_ARG;
     }

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "../../include/pcw.h"
 
 namespace pcw {
@@ -52,6 +54,16 @@ struct KInst {
 
 // element offset of frame column i within a row
 __host__ __device__ inline int col_pos(const KInst& I, int i) { return col_pos(I.slot_rows, I.nslot, i); }
+
+// Tile height of an instance for a requested height ty.  The FP32 step
+// addresses a tile's frame rows y0-1 .. y1+3 through buffer descriptors with
+// 32-bit byte offsets, so those rows of one plane must span less than 2^31
+// bytes; the FP64 kernels index with 64-bit offsets.
+inline int tile_rows_for(const KInst& I, int ty) {
+  if (!I.slot_rows) return ty;
+  const long long cap = 0x7fffffffLL / (4LL * I.pitch) - 5;
+  return (int)std::max(1LL, std::min((long long)ty, cap));
+}
 
 // time_control fields (MOD_time_control.f90:16-49), FP64.
 struct TimeCtl {

@@ -101,6 +101,8 @@ extern "C" int pcw_step_noalloc(const pcw_params* p, const pcw_domain* d, const 
   if (p->higher_order && p->use_limiter && (p->limiter < 1 || p->limiter > 4))
     return fail(PCW_ERR_ARG, "invalid limiter choice => ABORTING");
   if (d->nx < 2 || d->ny < 2) return fail(PCW_ERR_ARG, "the domain needs nx >= 2 and ny >= 2");
+  if (d->nx > PCW_MAX_SIDE || d->ny > PCW_MAX_SIDE)
+    return fail(PCW_ERR_ARG, "the domain needs nx, ny <= PCW_MAX_SIDE (262,136)");
   for (int s = 0; s < 4; ++s)
     if (d->bc[s] < PCW_BC_ZERO_GRADIENT || d->bc[s] > PCW_BC_REFLECTIVE)
       return fail(PCW_ERR_BC, d->bc[s] == 1 ? "User-defined boundary condition not implemented yet!"
@@ -118,7 +120,7 @@ extern "C" int pcw_step_noalloc(const pcw_params* p, const pcw_domain* d, const 
   if (diff > -frame_bytes && diff < frame_bytes) return fail(PCW_ERR_ARG, "state_in and state_out overlap");
   I.off[0] = 0;
   I.off[1] = diff / esize;
-  const int ty = p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128);
+  const int ty = tile_rows_for(I, p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128));
   const int nsx = (d->nx + kStripOut - 1) / kStripOut;
   const int nb = (d->ny + ty - 1) / ty;
   const long long ntiles = (long long)nsx * nb;

@@ -68,6 +68,24 @@ def test_create_rejects_bad_arguments_before_touching_the_gpu():
     assert e.value.code == abi.PCW_ERR_ARG
 
 
+def test_create_rejects_sizes_beyond_the_limits_before_touching_the_gpu():
+    """nx, ny above PCW_MAX_SIDE and more than PCW_MAX_DOMAINS instances are
+    refused with a message (the values match the header's)."""
+    from paraclaws_amd import abi
+
+    text = open(HEADER).read()
+    assert int(re.search(r"#define PCW_MAX_SIDE (\d+)", text).group(1)) == abi.PCW_MAX_SIDE
+    assert int(re.search(r"#define PCW_MAX_DOMAINS (\d+)", text).group(1)) == abi.PCW_MAX_DOMAINS
+    for dom in (abi.Domain(abi.PCW_MAX_SIDE + 1, 16), abi.Domain(16, abi.PCW_MAX_SIDE + 1)):
+        for prec in (abi.FP64, abi.FP32):
+            with pytest.raises(abi.PcwError, match="PCW_MAX_SIDE") as e:
+                abi.Engine([abi.Domain(16, 16), dom], precision=prec)
+            assert e.value.code == abi.PCW_ERR_ARG
+    with pytest.raises(abi.PcwError, match="PCW_MAX_DOMAINS") as e:
+        abi.Engine([abi.Domain(4, 4)] * (abi.PCW_MAX_DOMAINS + 1))
+    assert e.value.code == abi.PCW_ERR_ARG
+
+
 def test_missing_library_fails_loudly(tmp_path):
     from paraclaws_amd import abi
 
@@ -106,6 +124,8 @@ def test_step_noalloc_rejects_bad_arguments_before_touching_the_gpu():
     assert rc == abi.PCW_ERR_ARG and "overlap" in msg
     rc, _, msg = abi.step_noalloc(p, abi.Domain(64, 64, bc=(1, 2, 2, 2)), 0x10000, 0x900000, 1e-3)
     assert rc == abi.PCW_ERR_BC
+    rc, _, msg = abi.step_noalloc(p, abi.Domain(64, abi.PCW_MAX_SIDE + 1), 0x10000, 0x900000, 1e-3)
+    assert rc == abi.PCW_ERR_ARG and "PCW_MAX_SIDE" in msg
     rc, _, msg = abi.step_noalloc(abi.make_params(abi.FP64, tile_rows=2), abi.Domain(60 * 600, 1000),
                                   0x10000000, 0x80000000000, 1e-3)
     assert rc == abi.PCW_ERR_ARG and "tiles" in msg

@@ -199,3 +199,35 @@ def test_max_size_16384_fused_equals_staged():
     for name, (a, b) in masses.items():
         assert abs(b - a) <= (1e-12 if name != "fp32" else 1e-6) * a, (name, a, b)
     assert np.abs(states["fused"][2:-2, 2:-2, 1]).max() > 0.0
+
+
+def test_fp32_tile_height_capped_at_32bit_buffer_offsets():
+    """FP32 tiles address their rows through 32-bit buffer offsets, so a tile
+    spans less than 2^31 bytes of a plane.  16384 x 32768 (5.4e8 cells, a
+    plane of 2.3e9 bytes) with tile_rows = 32768 asks for one band over the
+    whole domain: the band is lowered to fit (two bands), and three accepted
+    steps under the device controller end bit for bit where the default
+    64-row tiles end."""
+    abi = _abi()
+    from paraclaws_amd import ic
+
+    nx, ny = 16384, 32768
+    q0 = ic.dam_break(nx, ny, xi=(-2.0, 2.0), eta=(-2.0, 2.0))
+    out = {}
+    for rows in (32768, 0):
+        _say(f"{nx} x {ny} FP32 dam break, 3 accepted steps, tile_rows {rows}")
+        with abi.Engine([abi.Domain(nx, ny, bc=(4, 4, 4, 4), xi=(-2.0, 2.0), eta=(-2.0, 2.0))],
+                        precision=abi.FP32, limiter=4, tile_rows=rows) as e:
+            e.set_state(0, q0)
+            e.fill_ghosts()
+            e.time_init(**TKW)
+            e.run(steps_per_graph=3, max_accepted=3)
+            st = e.status(0)
+            assert st.status == abi.STATUS_DONE and st.accepted == 3 and st.rejected == 0
+            out[rows] = (st.time, st.dt, e.get_state(0))
+    del q0
+    _say("tile height cap: comparing")
+    (ta, da, a), (tb, db, b) = out[32768], out[0]
+    assert ta == tb and da == db
+    assert np.array_equal(a[2:-2].view(np.uint64), b[2:-2].view(np.uint64))
+    assert np.abs(a[2:-2, 2:-2, 1]).max() > 0.0
