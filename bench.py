@@ -64,13 +64,21 @@ TIME_KW = dict(t_final=0.5, dt0=1e-10, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10
 
 
 def run_engine(abi, ic, n, precision, engine, steps, warmup, spg, autotune, log_steps=0,
-               repeat=1, tiles=None):
+               repeat=1, tiles=None, block=None):
     """Dam break n^2 from the initial state: `warmup` untimed accepted steps,
     then `repeat` timed runs of exactly `steps` accepted steps, each from the
     initial state.  The returned record is the run with the median wall time
-    (walls of all runs in `walls_s`); the engine holds that run's state."""
-    dom = abi.Domain(n, n, bc=(4, 4, 4, 4))
-    q0 = ic.dam_break(n, n)
+    (walls of all runs in `walls_s`); the engine holds that run's state.
+    block = (nx, ny): the lower-left nx x ny block of the n^2 dam break on
+    [-1,1]^2 instead (walls on its four sides)."""
+    if block:
+        bx, by = block
+        xi, eta = (-1.0, -1.0 + 2.0 * bx / n), (-1.0, -1.0 + 2.0 * by / n)
+        dom = abi.Domain(bx, by, bc=(4, 4, 4, 4), xi=xi, eta=eta)
+        q0 = ic.dam_break(bx, by, xi=xi, eta=eta)
+    else:
+        dom = abi.Domain(n, n, bc=(4, 4, 4, 4))
+        q0 = ic.dam_break(n, n)
     e = abi.Engine([dom], precision=precision, engine=engine, limiter=4,
                    **(dict(tile_rows=tiles[0], waves_per_block=tiles[1]) if tiles else {}))
 
@@ -694,6 +702,24 @@ def main():
                                   "roofline_frac_24B": BYTES_PER_CELL_UPDATE_F32 * 4096 * 4096
                                   / (r4["avg_step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                   "tuned": r4["tuned"]}
+        # one GPU's share of the n^2 domain in the block decompositions a
+        # multi-GPU run would use (2: 1x2, 4: 2x2, 8: 4x2 blocks), the walls
+        # standing in for the halo exchange
+        blocks = {}
+        for g, (bx, by) in ((2, (n, n // 2)), (4, (n // 2, n // 2)), (8, (n // 4, n // 2))):
+            log(f"FP32 fused engine, one GPU's block of {n}^2 over {g} GPUs: {bx} x {by}")
+            eb, rb = run_engine(abi, ic, n, abi.FP32, abi.FUSED, args.steps, args.warmup,
+                                args.steps_per_graph, not args.no_autotune, repeat=args.repeat,
+                                block=(bx, by))
+            eb.close()
+            blocks[str(g)] = {"block": [bx, by], "value": rb["accepted"] * bx * by / rb["wall_s"],
+                              "unit": "cell-updates/s", "kernel_avg_ms": rb["avg_step_ms"],
+                              "ms_per_step": rb["wall_s"] * 1e3 / max(1, rb["accepted"]),
+                              "roofline_frac_24B": BYTES_PER_CELL_UPDATE_F32 * bx * by
+                              / (rb["avg_step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                              "tuned": rb["tuned"]}
+            log(f"  {blocks[str(g)]}")
+        out_extra["fp32_rank_blocks"] = blocks
     if not args.no_fp64:
         log(f"FP64 plain (staged) engine, same workload")
         e64, r64 = run_engine(abi, ic, n, abi.FP64, abi.STAGED, args.steps, min(args.warmup, 5),
