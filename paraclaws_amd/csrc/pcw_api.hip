@@ -209,10 +209,10 @@ void order_tiles(pcw_handle* h) {
 }
 
 // Row bands [y0, y1] covering 1..ny: bands of ty rows, or, when short_ends,
-// first and last bands of about two thirds of ty (short_all: every band).
+// first and last bands of about four fifths of ty (short_all: every band).
 void row_bands(int ny, int ty, bool short_ends, bool short_all, std::vector<int2>& out) {
   out.clear();
-  const int te = std::max(1, (2 * ty + 2) / 3);
+  const int te = std::max(1, (4 * ty + 2) / 5);
   if (short_all) ty = te;
   if (!short_ends || short_all || ny <= 2 * te + ty) {
     const int nb = (ny + ty - 1) / ty;
@@ -231,8 +231,8 @@ void row_bands(int ny, int ty, bool short_ends, bool short_all, std::vector<int2
 
 // Tiles of every instance: strips of kStripOut columns (the last one shifted
 // to end at column nx) times row bands, ordered by band start then column.
-// FP32: tiles on a domain boundary run the kernel's general body, about 1.5x
-// an interior tile's time per row, so they get about two thirds of the rows
+// FP32: tiles on a domain boundary run the kernel's general body, about 1.2x
+// an interior tile's time per row, so they get about four fifths of the rows
 // (every band of the two edge strips, the first and last band of the others):
 // a launch then does not wait on its boundary tiles.
 // number of tiles build_tiles makes for tile height ty
