@@ -76,6 +76,24 @@ __device__ __forceinline__ void tile_finish(InstState* S, const KInst& I, int cu
   finish_attempt(S, I, cur, sx, sy, badbits);
 }
 
+// FP32 engine: read and reset the instance accumulators, complete the attempt
+__device__ __forceinline__ void complete_f32(InstState* S, const KInst& I, int cur) {
+  const float sx = __builtin_bit_cast(float, atomicExch(&S->smax32_x, 0u));
+  const float sy = __builtin_bit_cast(float, atomicExch(&S->smax32_y, 0u));
+  int bad = atomicExch(&S->bad, 0);
+  atomicExch(&S->tiles_done, 0u);
+  if (sx != sx) bad |= 1;  // a NaN speed (NaN momentum) is reported like a NaN Jacobian
+  if (sy != sy) bad |= 2;
+  finish_attempt(S, I, cur, (double)sx, (double)sy, bad);
+}
+
+// Instances of at most kDirectTiles tiles complete on one level: the two
+// extra dependent atomic round trips of the second level cost about 1 us per
+// step, more than that many tiles' atomics on one address (at 256 the
+// 512-instance ensemble, most of whose instances are that small, ran 0.4 %
+// slower).
+constexpr int kDirectTiles = 128;
+
 // Two-level completion (FP32 engine, many tiles per instance): tile t of the
 // instance (t = 0 .. ntiles-1) publishes its maxima (float bits; as unsigned
 // integers a NaN wins) and error bits to slot t % kSlots, waits for those
@@ -86,6 +104,23 @@ __device__ __forceinline__ void tile_finish(InstState* S, const KInst& I, int cu
 __device__ __forceinline__ void tile_finish_slots(InstState* S, const KInst& I, TileSlot* slots,
                                                   int t, int cur, float mx, float my, int badbits,
                                                   int lane) {
+  if (I.ntiles <= kDirectTiles) {
+    // few tiles: one level, every tile on the instance's accumulators
+    unsigned int prev = 0;
+    if (lane == 0) {
+      const unsigned o1 = atomicMax(&S->smax32_x, __builtin_bit_cast(unsigned, mx));
+      const unsigned o2 = atomicMax(&S->smax32_y, __builtin_bit_cast(unsigned, my));
+      int o3 = 0;
+      if (badbits) o3 = atomicOr(&S->bad, badbits);
+      asm volatile("" ::"v"(o1), "v"(o2), "v"(o3));  // wait until the three have completed
+      prev = atomicAdd(&S->tiles_done, 1u);
+    }
+    prev = __shfl(prev, 0, 64);
+    if (prev + 1 != (unsigned)I.ntiles) return;
+    if (lane != 0) return;
+    complete_f32(S, I, cur);
+    return;
+  }
   const int nslots = I.ntiles < kSlots ? I.ntiles : kSlots;
   const int s = t % kSlots;
   const unsigned per = (unsigned)(I.ntiles / kSlots + (s < I.ntiles % kSlots ? 1 : 0));
@@ -113,14 +148,7 @@ __device__ __forceinline__ void tile_finish_slots(InstState* S, const KInst& I, 
   if (sbad) p3 = atomicOr(&S->bad, sbad);
   asm volatile("" ::"v"(p1), "v"(p2), "v"(p3));
   if (atomicAdd(&S->tiles_done, 1u) + 1 != (unsigned)nslots) return;
-  // last slot: complete the attempt
-  const float sx = __builtin_bit_cast(float, atomicExch(&S->smax32_x, 0u));
-  const float sy = __builtin_bit_cast(float, atomicExch(&S->smax32_y, 0u));
-  int bad = atomicExch(&S->bad, 0);
-  atomicExch(&S->tiles_done, 0u);
-  if (sx != sx) bad |= 1;  // a NaN speed (NaN momentum) is reported like a NaN Jacobian
-  if (sy != sy) bad |= 2;
-  finish_attempt(S, I, cur, (double)sx, (double)sy, bad);
+  complete_f32(S, I, cur);  // last slot: complete the attempt
 }
 
 }  // namespace pcw
