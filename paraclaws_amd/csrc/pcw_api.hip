@@ -510,13 +510,13 @@ int ensure_full_tiles(pcw_handle* h) {
 // Between the launches of a device-controlled run of several instances: when
 // instances have finished, the tile list is rebuilt for the running ones, so
 // that no launch dispatches tiles that exit at once, and with shorter tiles
-// (down to kMinRetileRows) while the running instances' tiles would not fill
-// the GPU's resident waves once (kResidentWavesPerCU per CU).  The step
-// launches read the list's length from device memory, so a captured graph
-// stays valid; it is captured again, for a smaller grid, once the list is
-// below a quarter of the grid.  Cell values do not depend on the tiling.
+// (shrink_rows) while the running instances' tiles would not fill the GPU.
+// The step launches read the list's length from device memory, so a captured
+// graph stays valid; it is captured again, for a smaller grid, once the list
+// is below a quarter of the grid.  Cell values do not depend on the tiling.
 constexpr int kMinRetileRows = 16;
 constexpr int kResidentWavesPerCU = 16;
+constexpr int kSimdsPerCU = 4;
 
 // compute units of the current device (the one the handle's memory is on)
 int cu_count() {
@@ -524,6 +524,25 @@ int cu_count() {
   if (hipGetDevice(&dev) != hipSuccess) return 1;
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
   return cu > 0 ? cu : 1;
+}
+
+// Tile height for ty requested: bands shortened by quarters down to
+// kMinRetileRows while the tiles would not fill the GPU's resident waves
+// once (kResidentWavesPerCU per CU), then further, down to 2 rows (FP32) or
+// 4 (FP64), while they would not give every SIMD one wave.  Such a small
+// domain's step is bound by one wave's march down its tile (about 0.7 us a
+// row in FP32), which shorter bands cut by more than the halo rows they
+// recompute cost (256^2: FP32 20.5 -> 13.2 us, FP64 52.2 -> 31.3 us per
+// step); FP64 stops at 4 rows, where its arithmetic for the recomputed rows
+// starts to dominate.  only: the instances to count (all when null).
+int shrink_rows(const pcw_handle* h, int ty, const std::vector<char>* only = nullptr) {
+  const long long cu = cu_count();
+  while (ty > kMinRetileRows && count_tiles(h, ty, only) < cu * kResidentWavesPerCU)
+    ty = std::max(kMinRetileRows, ty * 3 / 4);
+  const int floor_rows = h->p.precision == PCW_FP32 ? 2 : 4;
+  while (ty > floor_rows && count_tiles(h, ty, only) < cu * kSimdsPerCU)
+    ty = std::max(floor_rows, ty * 3 / 4);
+  return ty;
 }
 
 int maybe_retile(pcw_handle* h) {
@@ -535,10 +554,7 @@ int maybe_retile(pcw_handle* h) {
     nrun += run[k];
   }
   if (nrun == 0 || nrun == h->tiled_running) return PCW_OK;
-  const long long target = (long long)cu_count() * kResidentWavesPerCU;
-  int ty = h->tile_rows;
-  while (ty > kMinRetileRows && count_tiles(h, ty, &run) < target)
-    ty = std::max(kMinRetileRows, ty * 3 / 4);
+  int ty = shrink_rows(h, h->tile_rows, &run);
   while (count_tiles(h, ty, &run) > h->tile_cap) ty += std::max(1, ty / 4);
   build_tiles(h, ty, &run);
   h->tiles_subset = true;
@@ -641,16 +657,12 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
   h->bpool_elems = bbase;
   // default tile heights: FP32 64 rows (memory-bound; the autotuner refines it), FP64
   // 128 rows (arithmetic-bound: fewer recomputed halo rows per tile).  Without a
-  // height from the caller, the bands are shortened (down to kMinRetileRows) while
-  // the tiles would not fill the GPU's resident waves once: a small domain stepped
-  // without the autotuner (the Fortran host, pcw_step) then still spreads over the
-  // CUs (256^2: 80-100 tiles instead of 10-20).  Cell values do not depend on it.
+  // height from the caller, the bands are shortened (shrink_rows) while the tiles
+  // would not fill the GPU: a small domain stepped without the autotuner (the
+  // Fortran host, pcw_step) then still spreads over the CUs (256^2 in FP32: 640
+  // tiles of 2 rows instead of 10 of 64).  Cell values do not depend on it.
   int ty = p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128);
-  if (p->tile_rows <= 1) {
-    const long long target = (long long)cu_count() * kResidentWavesPerCU;
-    while (ty > kMinRetileRows && count_tiles(h, ty) < target)
-      ty = std::max(kMinRetileRows, ty * 3 / 4);
-  }
+  if (p->tile_rows <= 1) ty = shrink_rows(h, ty);
   h->tile_rows = ty;
   h->wpb = (p->waves_per_block >= 1 && p->waves_per_block <= 4) ? p->waves_per_block : 2;
   HIPCHK(hipStreamCreate(&h->stream));
@@ -1126,8 +1138,13 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
     any_src = any_src || d.source != PCW_SRC_NONE;
     any_nosrc = any_nosrc || (d.source == PCW_SRC_NONE && !(h->p.dry_tol > 0.0));
   }
+  // heights below 16 rows where even 16-row tiles would not give every SIMD a
+  // wave (a small domain, whose step one wave's march bounds)
+  const bool small = count_tiles(h, 16) < (long long)cu * kSimdsPerCU;
   for (int wv : wpb_c) {
     for (int tr : {16, 32, 64, 128, 256}) cand.push_back({tr, wv});
+    if (small)
+      for (int tr : {2, 4, 8}) cand.push_back({tr, wv});
     if (h->p.precision != PCW_FP32 || cu <= 0 || (any_src && any_nosrc) || (h->any_met && h->any_plain))
       continue;
     const int bpc = fused_f32_blocks_per_cu(limiter_code(h->p), h->p.higher_order != 0, h->any_met,
