@@ -210,9 +210,14 @@ void order_tiles(pcw_handle* h) {
 
 // Row bands [y0, y1] covering 1..ny: bands of ty rows, or, when short_ends,
 // first and last bands of about four fifths of ty (short_all: every band).
+// Heights below 3 are raised to 3, so that every band, and in particular the
+// first and the last, holds at least two rows (ny >= 2): the FP32 step
+// writes the eta ghost rows from the two rows next to the boundary, which
+// must be finished by the same tile.
 void row_bands(int ny, int ty, bool short_ends, bool short_all, std::vector<int2>& out) {
   out.clear();
-  const int te = std::max(1, (4 * ty + 2) / 5);
+  ty = std::max(ty, kMinBandRows);
+  const int te = std::max(kMinBandRows, (4 * ty + 2) / 5);
   if (short_all) ty = te;
   if (!short_ends || short_all || ny <= 2 * te + ty) {
     const int nb = (ny + ty - 1) / ty;
@@ -528,18 +533,18 @@ int cu_count() {
 
 // Tile height for ty requested: bands shortened by quarters down to
 // kMinRetileRows while the tiles would not fill the GPU's resident waves
-// once (kResidentWavesPerCU per CU), then further, down to 2 rows (FP32) or
+// once (kResidentWavesPerCU per CU), then further, down to 3 rows (FP32) or
 // 4 (FP64), while they would not give every SIMD one wave.  Such a small
 // domain's step is bound by one wave's march down its tile (about 0.7 us a
 // row in FP32), which shorter bands cut by more than the halo rows they
-// recompute cost (256^2: FP32 20.5 -> 13.2 us, FP64 52.2 -> 31.3 us per
-// step); FP64 stops at 4 rows, where its arithmetic for the recomputed rows
-// starts to dominate.  only: the instances to count (all when null).
+// recompute cost (256^2: FP32 20.5 -> 14 us, FP64 52.2 -> 31.3 us per step);
+// FP64 stops at 4 rows, where its arithmetic for the recomputed rows starts
+// to dominate.  only: the instances to count (all when null).
 int shrink_rows(const pcw_handle* h, int ty, const std::vector<char>* only = nullptr) {
   const long long cu = cu_count();
   while (ty > kMinRetileRows && count_tiles(h, ty, only) < cu * kResidentWavesPerCU)
     ty = std::max(kMinRetileRows, ty * 3 / 4);
-  const int floor_rows = h->p.precision == PCW_FP32 ? 2 : 4;
+  const int floor_rows = h->p.precision == PCW_FP32 ? kMinBandRows : 4;
   while (ty > floor_rows && count_tiles(h, ty, only) < cu * kSimdsPerCU)
     ty = std::max(floor_rows, ty * 3 / 4);
   return ty;
@@ -659,8 +664,8 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
   // 128 rows (arithmetic-bound: fewer recomputed halo rows per tile).  Without a
   // height from the caller, the bands are shortened (shrink_rows) while the tiles
   // would not fill the GPU: a small domain stepped without the autotuner (the
-  // Fortran host, pcw_step) then still spreads over the CUs (256^2 in FP32: 640
-  // tiles of 2 rows instead of 10 of 64).  Cell values do not depend on it.
+  // Fortran host, pcw_step) then still spreads over the CUs (256^2 in FP32: 430
+  // tiles of 3 rows or fewer instead of 10 of 64).  Cell values do not depend on it.
   int ty = p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128);
   if (p->tile_rows <= 1) ty = shrink_rows(h, ty);
   h->tile_rows = ty;
@@ -1144,7 +1149,7 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
   for (int wv : wpb_c) {
     for (int tr : {16, 32, 64, 128, 256}) cand.push_back({tr, wv});
     if (small)
-      for (int tr : {2, 4, 8}) cand.push_back({tr, wv});
+      for (int tr : {kMinBandRows, 4, 6, 8}) cand.push_back({tr, wv});
     if (h->p.precision != PCW_FP32 || cu <= 0 || (any_src && any_nosrc) || (h->any_met && h->any_plain))
       continue;
     const int bpc = fused_f32_blocks_per_cu(limiter_code(h->p), h->p.higher_order != 0, h->any_met,

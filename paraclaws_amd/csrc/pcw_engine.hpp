@@ -14,6 +14,7 @@
 namespace pcw {
 
 constexpr int kStripOut = 60;   // output columns per wave (64 lanes - 2x2 halo)
+constexpr int kMinBandRows = 3; // smallest requested band height (bands of >= 2 rows)
 
 // Layout of a frame row, columns i = -1 .. nx+2 (ghost, interior, ghost).
 // Plain rows (FP64 engines): column i at element i+1.  Slot rows (FP32

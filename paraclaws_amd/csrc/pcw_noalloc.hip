@@ -120,7 +120,10 @@ extern "C" int pcw_step_noalloc(const pcw_params* p, const pcw_domain* d, const 
   if (diff > -frame_bytes && diff < frame_bytes) return fail(PCW_ERR_ARG, "state_in and state_out overlap");
   I.off[0] = 0;
   I.off[1] = diff / esize;
-  const int ty = tile_rows_for(I, p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128));
+  // bands of at least two rows (the FP32 step finishes the eta ghost rows from
+  // the two rows next to the boundary within one tile)
+  const int ty = std::max(kMinBandRows,
+                          tile_rows_for(I, p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128)));
   const int nsx = (d->nx + kStripOut - 1) / kStripOut;
   const int nb = (d->ny + ty - 1) / ty;
   const long long ntiles = (long long)nsx * nb;
