@@ -372,7 +372,8 @@ int enqueue_fused(pcw_handle* h) {
     hipError_t e = hipSuccess;
     for (int phys = 0; phys <= 1 && e == hipSuccess; ++phys) {
       if (phys ? !any_src : !any_nosrc) continue;
-      e = launch_fused_f32(limiter_code(h->p), met, phys, a, blocks, threads, h->stream);
+      e = launch_fused_f32(limiter_code(h->p), met, phys ? (wetdry ? 2 : 1) : 0, a, blocks, threads,
+                           h->stream);
     }
     if (e != hipSuccess)
       return fail(h, PCW_ERR_HIP, std::string("fused launch: ") + hipGetErrorString(e));
@@ -1162,7 +1163,7 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
     if (h->p.precision != PCW_FP32 || cu <= 0 || (any_src && any_nosrc) || (h->any_met && h->any_plain))
       continue;
     const int bpc = fused_f32_blocks_per_cu(limiter_code(h->p), h->p.higher_order != 0, h->any_met,
-                                            any_src, 64 * wv);
+                                            any_src ? (h->p.dry_tol > 0.0 ? 2 : 1) : 0, 64 * wv);
     if (bpc <= 0) continue;
     const long long slots = (long long)bpc * wv * cu;  // resident waves
     auto ntiles = [&](int tr) { return count_tiles(h, tr); };

@@ -278,7 +278,7 @@ __device__ __forceinline__ void source(float& h, float& hu, float& hv, int kind,
   hv = nhv;
 }
 
-template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
+template <int LIM, bool HO, bool MET, int PHYS, bool GEN>
 __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const Slot& A,
                                        const Slot& B, float acc[3], float bm, float b0,
                                        float bnew, float finm[3]);
@@ -286,18 +286,18 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
 // Input rows are prefetched kPF rows ahead, through a ring of kPF buffers
 // (6 for the plain variants; 3 where metric arrays or source terms take the
 // registers).
-template <bool MET, bool PHYS>
+template <bool MET, int PHYS>
 struct Ring {
   static constexpr int kPF = (MET || PHYS) ? 3 : 6;
 };
 
 // xi sweep of row r: the xi-updated state qx of the row's cells from the
 // prefetched input P (P itself for rows outside 1..ny in the general body).
-template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
+template <int LIM, bool HO, bool MET, int PHYS, bool GEN>
 __device__ __forceinline__ void xi_row(const Ctx& C, int r, const Pre& P, Red& R, float qx[3]) {
   const float q[3] = {P.h, P.hu, P.hv};
   const float mk = P.k, mgx = P.gx;
-  const bool wet = PHYS && C.wetdry;
+  constexpr bool wet = PHYS == 2;
   qx[0] = q[0];
   qx[1] = q[1];
   qx[2] = q[2];
@@ -310,7 +310,7 @@ __device__ __forceinline__ void xi_row(const Ctx& C, int r, const Pre& P, Red& R
     const float fR[3] = {q[1], f1, f2};
     const float sL = lft(C, s);
     float w1[3], w2[3];
-    const float s2 = hll<PHYS>(qL, fL, sL, q, fR, s, C.eps, w1, w2);
+    const float s2 = hll<(PHYS != 0)>(qL, fL, sL, q, fR, s, C.eps, w1, w2);
     R.smx = smax2(R.smx, (!GEN || C.xface) ? s2 : 0.0f);
     float p1 = 0.5f, p2 = 0.5f;  // halved limiter values (unlimited: 1 / 2)
     if (HO && LIM > 0) {
@@ -362,7 +362,7 @@ __device__ __forceinline__ void xi_row(const Ctx& C, int r, const Pre& P, Red& R
 }
 
 // load row r+kPF (clamped to the tile's last row) into P
-template <bool MET, bool PHYS>
+template <bool MET, int PHYS>
 __device__ __forceinline__ void prefetch(const Ctx& C, int r, Pre& P) {
   const int so = (min(r + Ring<MET, PHYS>::kPF, C.y1 + 2) + 1 - C.rbase) * C.p4;
   P.h = bload(C.qi[0], C.lo, so);
@@ -378,11 +378,11 @@ __device__ __forceinline__ void prefetch(const Ctx& C, int r, Pre& P) {
 // eta sweep of row r from its xi-updated state qx: the row's eta cell
 // values into slot N, the face below it (rows r-1 | r), and from row y0+1
 // on the finishing of face r-1 (finish).  mk, mgy: the row's metric values.
-template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
+template <int LIM, bool HO, bool MET, int PHYS, bool GEN>
 __device__ __forceinline__ void eta_row(const Ctx& C, int r, const float qx[3], float mk,
                                         float mgy, Slot& N, Slot& A, Slot& B, float acc[3],
                                         Red& R, float& bm, float& b0, float bnew, float finm[3]) {
-  const bool wet = PHYS && C.wetdry;
+  constexpr bool wet = PHYS == 2;
 #pragma unroll
   for (int k = 0; k < 3; ++k) N.q[k] = qx[k];
   if (wet) cell<2, true>(qx[0], qx[1], qx[2], C.g, C.dry, N.s, N.f1, N.f2);
@@ -395,7 +395,7 @@ __device__ __forceinline__ void eta_row(const Ctx& C, int r, const float qx[3], 
   // interface r (rows r-1 | r)
   if (!GEN || r >= C.y0 - 1) {
     const float fA[3] = {A.q[2], A.f1, A.f2}, fN[3] = {qx[2], N.f1, N.f2};
-    N.s2 = hll<PHYS>(A.q, fA, A.s, qx, fN, N.s, C.eps, N.w1, N.w2);
+    N.s2 = hll<(PHYS != 0)>(A.q, fA, A.s, qx, fN, N.s, C.eps, N.w1, N.w2);
     R.smy = smax2(R.smy, (!GEN || (r >= 0 && r <= C.ny + 1)) ? N.s2 : 0.0f);
   } else {
     N.s2 = 0.0f;
@@ -415,7 +415,7 @@ __device__ __forceinline__ void eta_row(const Ctx& C, int r, const float qx[3], 
 // reloaded with row r+kPF.  GEN: the general body with every range test;
 // otherwise the row must be an interior row of an interior strip (see the
 // kernel's chunk test).
-template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
+template <int LIM, bool HO, bool MET, int PHYS, bool GEN>
 __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot& B, Pre& P,
                                     float acc[3], Red& R, float& bm, float& b0, float finm[3]) {
   // rows are not interleaved by the scheduler: that would move the
@@ -445,7 +445,7 @@ __device__ __forceinline__ void row(const Ctx& C, int r, Slot& N, Slot& A, Slot&
 // The part of a row that needs interface r-1 (from row y0+1 on): its
 // limiter and correction flux, row r-2's last contribution, source term,
 // stores and ghosts, and row r-1's first contribution.
-template <int LIM, bool HO, bool MET, bool PHYS, bool GEN>
+template <int LIM, bool HO, bool MET, int PHYS, bool GEN>
 __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const Slot& A,
                                        const Slot& B, float acc[3], float bm, float b0,
                                        float bnew, float finm[3]) {
@@ -488,7 +488,7 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
     float qn[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) qn[k] = PS ? acc[k] - dH[k] : fmaf(-facB, dH[k], acc[k]);
-    if (PHYS && (C.src != PCW_SRC_NONE || C.wetdry)) {
+    if (PHYS == 2 || (PHYS == 1 && C.src != PCW_SRC_NONE)) {
       const float bx = shl1(b0) - shr1(b0), by = bnew - bm;
       source(qn[0], qn[1], qn[2], C.src, C.dry, C.g, C.gdx, C.gdy, C.dt, C.n2, bx, by);
     }
@@ -558,7 +558,7 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
 // Four waves per SIMD except with metric arrays: the source-term variant
 // with the MC limiter needs 129 VGPRs unbounded (three waves); bounded to 128
 // it spills 8 bytes per lane and runs config 3's step 2.7 % faster.
-template <int LIM, bool HO, bool MET, bool PHYS>
+template <int LIM, bool HO, bool MET, int PHYS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MET ? 1 : 4)))
 k_step(FusedArgs<float> a) {
   const int lane = threadIdx.x & 63;
@@ -579,7 +579,7 @@ k_step(FusedArgs<float> a) {
   if (S->status != PCW_STATUS_RUNNING) return;
   if (MET != (I.moff >= 0)) return;  // instances with and without metric arrays run in separate launches
   const bool phys_inst = I.source != PCW_SRC_NONE || a.wetdry;
-  if (PHYS != phys_inst) return;      // likewise instances with and without source terms
+  if ((PHYS != 0) != phys_inst) return;  // likewise instances with and without source terms
 
   Ctx C;
   const int cur = S->cur;
@@ -718,14 +718,14 @@ k_step(FusedArgs<float> a) {
 
   // wave maxima and error bits; the instance's last tile completes the attempt
   const float mx = wave_smax(R.smx), my = wave_smax(C.out_lane ? R.smy : 0.0f);
-  const bool chk = !(PHYS && a.wetdry);  // with the wet/dry cut-off h <= 0 is a dry cell
+  const bool chk = PHYS != 2;  // with the wet/dry cut-off h <= 0 is a dry cell
   const int bad = chk ? (__builtin_bit_cast(unsigned, mx) >= kNotFinite ? 1 : 0) |
                             (__builtin_bit_cast(unsigned, my) >= kNotFinite ? 2 : 0)
                       : 0;
   tile_finish_slots(S, I, a.slots + (size_t)in * kSlots, tid - I.tile0, cur, mx, my, bad, lane);
 }
 
-template <bool HO, bool MET, bool PHYS>
+template <bool HO, bool MET, int PHYS>
 static hipError_t launch_lim(int lim, const FusedArgs<float>& a, int blocks, int threads,
                              hipStream_t s) {
   switch (lim) {
@@ -738,7 +738,7 @@ static hipError_t launch_lim(int lim, const FusedArgs<float>& a, int blocks, int
   return hipGetLastError();
 }
 
-template <bool MET, bool PHYS>
+template <bool MET, int PHYS>
 static hipError_t launch_ho(int lim, const FusedArgs<float>& a, int blocks, int threads,
                             hipStream_t s) {
   if (!a.ho) {
@@ -748,7 +748,7 @@ static hipError_t launch_ho(int lim, const FusedArgs<float>& a, int blocks, int 
   return launch_lim<true, MET, PHYS>(lim, a, blocks, threads, s);
 }
 
-template <bool MET, bool PHYS>
+template <bool MET, int PHYS>
 static int occupancy_ho(int lim, bool ho, int threads) {
   int n = 0;
   hipError_t e;
@@ -768,21 +768,26 @@ static int occupancy_ho(int lim, bool ho, int threads) {
 
 }  // namespace f32
 
-int fused_f32_blocks_per_cu(int lim, bool ho, bool met, bool phys, int threads) {
-  if (met) return phys ? f32::occupancy_ho<true, true>(lim, ho, threads)
-                       : f32::occupancy_ho<true, false>(lim, ho, threads);
-  return phys ? f32::occupancy_ho<false, true>(lim, ho, threads)
-              : f32::occupancy_ho<false, false>(lim, ho, threads);
+// phys: 0 no source terms, 1 source terms without the wet/dry cut-off, 2 with it
+int fused_f32_blocks_per_cu(int lim, bool ho, bool met, int phys, int threads) {
+  if (met) return phys == 2 ? f32::occupancy_ho<true, 2>(lim, ho, threads)
+                  : phys == 1 ? f32::occupancy_ho<true, 1>(lim, ho, threads)
+                              : f32::occupancy_ho<true, 0>(lim, ho, threads);
+  return phys == 2 ? f32::occupancy_ho<false, 2>(lim, ho, threads)
+         : phys == 1 ? f32::occupancy_ho<false, 1>(lim, ho, threads)
+                     : f32::occupancy_ho<false, 0>(lim, ho, threads);
 }
 
-hipError_t launch_fused_f32(int lim, bool met, bool phys, const FusedArgs<float>& a, int blocks,
+hipError_t launch_fused_f32(int lim, bool met, int phys, const FusedArgs<float>& a, int blocks,
                             int threads, hipStream_t s) {
   if (met) {
-    return phys ? f32::launch_ho<true, true>(lim, a, blocks, threads, s)
-                : f32::launch_ho<true, false>(lim, a, blocks, threads, s);
+    return phys == 2 ? f32::launch_ho<true, 2>(lim, a, blocks, threads, s)
+           : phys == 1 ? f32::launch_ho<true, 1>(lim, a, blocks, threads, s)
+                       : f32::launch_ho<true, 0>(lim, a, blocks, threads, s);
   }
-  return phys ? f32::launch_ho<false, true>(lim, a, blocks, threads, s)
-              : f32::launch_ho<false, false>(lim, a, blocks, threads, s);
+  return phys == 2 ? f32::launch_ho<false, 2>(lim, a, blocks, threads, s)
+         : phys == 1 ? f32::launch_ho<false, 1>(lim, a, blocks, threads, s)
+                     : f32::launch_ho<false, 0>(lim, a, blocks, threads, s);
 }
 
 }  // namespace pcw

@@ -49,10 +49,12 @@ struct FusedArgs {
   TileSlot* slots;     // FP32: kSlots completion accumulators per instance
 };
 
-hipError_t launch_fused_f32(int lim, bool met, bool phys, const FusedArgs<float>& a, int blocks,
+// phys: 0 instances without source terms, 1 with source terms and without the
+// wet/dry cut-off, 2 with the wet/dry cut-off (every instance then)
+hipError_t launch_fused_f32(int lim, bool met, int phys, const FusedArgs<float>& a, int blocks,
                             int threads, hipStream_t s);
 // resident blocks per CU of the FP32 fused kernel variant (0 if unknown)
-int fused_f32_blocks_per_cu(int lim, bool ho, bool met, bool phys, int threads);
+int fused_f32_blocks_per_cu(int lim, bool ho, bool met, int phys, int threads);
 // Mesh metrics in an FP64 fused launch: identity-map instances with the
 // reference's rounded metrics computed in the kernel, instances with metric
 // arrays, and identity-map instances whose rounded metrics are all exactly 1

@@ -162,7 +162,8 @@ extern "C" int pcw_step_noalloc(const pcw_params* p, const pcw_domain* d, const 
     FusedArgs<float> a{static_cast<float*>(const_cast<void*>(state_in)), nullptr, nullptr, dinst, dst,
                        dtiles, dlive, (int)ntiles, p->higher_order, wetdry, 0, p->grav, p->eps,
                        p->dry_tol, dslots};
-    e = launch_fused_f32(lim, false, d->source != PCW_SRC_NONE || wetdry, a, blocks, 64 * wpb, s);
+    e = launch_fused_f32(lim, false, wetdry ? 2 : (d->source != PCW_SRC_NONE ? 1 : 0), a, blocks,
+                         64 * wpb, s);
   } else {
     FusedArgs<double> a{static_cast<double*>(const_cast<void*>(state_in)), nullptr, nullptr, dinst,
                         dst, dtiles, dlive, (int)ntiles, p->higher_order, wetdry, 0, p->grav, p->eps,
