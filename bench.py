@@ -40,7 +40,7 @@ BYTES_PER_CELL_UPDATE_F32 = 24.0
 BYTES_PER_CELL_UPDATE_F64 = 48.0
 # the FP32 fused step kernel of this workload (MC limiter, second order, no
 # metric arrays, no source terms)
-KERNEL_F32 = "pcw::f32::k_step<4, true, false, false>"
+KERNEL_F32 = "pcw::f32::k_step<4, true, false, 0>"
 METRIC = "cell-updates/s/GPU (FP32 batched vs FP64 plain) at stated error; % HBM peak"
 
 
