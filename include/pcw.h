@@ -172,8 +172,9 @@ int pcw_fill_ghosts(pcw_handle* h);
 
 /* One attempt of one domain on device frames the caller owns, allocating
  * nothing.  It has no counterpart in the reference: it is pcw_step for
- * callers that cannot allocate device memory (code under stream capture,
- * serving loops, harnesses that forbid hipMalloc).
+ * callers that cannot allocate device memory (serving loops, harnesses that
+ * forbid hipMalloc).  It waits on `stream` to return smax, so it cannot be
+ * recorded under stream capture.
  *   state_in, state_out: device frames (256-byte aligned, not overlapping) of
  *     three planes h, hu, hv, each ny+4 rows of `pitch` elements of the
  *     handle's layout: FP64 rows hold column i (-1 .. nx+2) at element i+1,

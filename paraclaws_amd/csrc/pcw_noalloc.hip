@@ -1,7 +1,8 @@
 // pcw_noalloc.hip -- pcw_step_noalloc: one attempt of one domain on frames
 // the caller owns, with no device allocation, for callers that cannot
-// allocate (code under stream capture, serving loops, harnesses that forbid
-// hipMalloc).
+// allocate (serving loops, harnesses that forbid hipMalloc).  It waits on
+// its stream to return the sweeps' speed maxima, so it is not for code under
+// stream capture.
 //
 // The step kernels of pcw_step need, besides the state, a descriptor, the
 // attempt's controller record, the completion accumulators (FP32) and the
