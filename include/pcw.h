@@ -128,7 +128,10 @@ typedef struct pcw_status {
  * above, nx or ny below 2 and bad settings; a handle is returned then too,
  * for pcw_last_error, and is released with pcw_destroy.  FP32 tiles span at
  * most the rows whose three planes' 32-bit buffer offsets stay below 2^31
- * bytes per plane: a larger tile_rows is lowered to that for the instance. */
+ * bytes per plane: a larger tile_rows is lowered to that for the instance.
+ * Threads: a handle is used by one host thread at a time; different handles
+ * may be driven from different threads at once.  Each handle has its own
+ * non-blocking HIP stream, and its transfers are ordered on that stream. */
 int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains,
                pcw_handle** out);
 int pcw_destroy(pcw_handle* h);

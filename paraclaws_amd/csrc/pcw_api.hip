@@ -164,6 +164,20 @@ int fail(pcw_handle* h, int code, const std::string& msg) {
 
 long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
 
+// Transfers and fills of a handle's memory go through the handle's own
+// stream, and the call waits for it.  That stream is created non-blocking, so
+// nothing here is ordered with the legacy (null) stream: handles used from
+// several host threads at once, one thread capturing a step graph while
+// another uploads a state, do not wait on or break each other.
+hipError_t copy_wait(pcw_handle* h, void* dst, const void* src, size_t n, hipMemcpyKind kind) {
+  const hipError_t e = hipMemcpyAsync(dst, src, n, kind, h->stream);
+  return e == hipSuccess ? hipStreamSynchronize(h->stream) : e;
+}
+hipError_t zero_wait(pcw_handle* h, void* p, size_t n) {
+  const hipError_t e = hipMemsetAsync(p, 0, n, h->stream);
+  return e == hipSuccess ? hipStreamSynchronize(h->stream) : e;
+}
+
 // A tile that touches a domain boundary (the first strip, the strip ending at
 // column nx, the first two or last two rows) runs the kernels' general body
 // and takes longer than an interior tile.
@@ -299,14 +313,15 @@ void drop_graph(pcw_handle* h);
 int upload_tiles(pcw_handle* h) {
   if (!h->dtiles || h->ntiles > h->dtiles_cap) {
     drop_graph(h);  // captured launches hold the old list's address
+    HIPCHK(hipStreamSynchronize(h->stream));  // and launched ones may still read it
     if (h->dtiles) HIPCHK(hipFree(h->dtiles));
     h->dtiles = nullptr;
     h->dtiles_cap = std::max(1, h->ntiles);
     HIPCHK(hipMalloc(&h->dtiles, sizeof(int4) * h->dtiles_cap));
   }
-  HIPCHK(hipMemcpy(h->dtiles, h->htiles.data(), sizeof(int4) * h->ntiles, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->dinst, h->hinst.data(), sizeof(KInst) * h->n, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->dlive, &h->ntiles, sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(copy_wait(h, h->dtiles, h->htiles.data(), sizeof(int4) * h->ntiles, hipMemcpyHostToDevice));
+  HIPCHK(copy_wait(h, h->dinst, h->hinst.data(), sizeof(KInst) * h->n, hipMemcpyHostToDevice));
+  HIPCHK(copy_wait(h, h->dlive, &h->ntiles, sizeof(int), hipMemcpyHostToDevice));
   return PCW_OK;
 }
 
@@ -680,27 +695,27 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
   if (p->tile_rows <= 1) ty = shrink_rows(h, ty);
   h->tile_rows = ty;
   h->wpb = (p->waves_per_block >= 1 && p->waves_per_block <= 4) ? p->waves_per_block : 2;
-  HIPCHK(hipStreamCreate(&h->stream));
+  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&h->ev0));
   HIPCHK(hipEventCreate(&h->ev1));
   HIPCHK(hipMalloc(&h->pool, (size_t)base * h->esize));
-  HIPCHK(hipMemset(h->pool, 0, (size_t)base * h->esize));
+  HIPCHK(zero_wait(h, h->pool, (size_t)base * h->esize));
   if (any_bathy) {
     HIPCHK(hipMalloc(&h->bpool, (size_t)bbase * h->esize));
-    HIPCHK(hipMemset(h->bpool, 0, (size_t)bbase * h->esize));
+    HIPCHK(zero_wait(h, h->bpool, (size_t)bbase * h->esize));
   }
   HIPCHK(hipMalloc(&h->dinst, sizeof(KInst) * n_domains));
   HIPCHK(hipMalloc(&h->dst, sizeof(InstState) * n_domains));
   HIPCHK(hipMalloc(&h->dscratch, sizeof(double) * kScratch));
   if (p->precision == PCW_FP32) {
     HIPCHK(hipMalloc(&h->dslots, sizeof(TileSlot) * kSlots * n_domains));
-    HIPCHK(hipMemset(h->dslots, 0, sizeof(TileSlot) * kSlots * n_domains));
+    HIPCHK(zero_wait(h, h->dslots, sizeof(TileSlot) * kSlots * n_domains));
   }
   if (p->engine == PCW_ENGINE_STAGED) {
     long long welems = 0;
     for (const auto& I : h->hinst) welems += (long long)W_COUNT * I.plane;
     HIPCHK(hipMalloc(&h->work, (size_t)welems * sizeof(double)));
-    HIPCHK(hipMemset(h->work, 0, (size_t)welems * sizeof(double)));
+    HIPCHK(zero_wait(h, h->work, (size_t)welems * sizeof(double)));
   }
   HIPCHK(hipMalloc(&h->dlive, sizeof(int)));
   build_tiles(h, h->tile_rows);
@@ -749,7 +764,7 @@ int pcw_set_state(pcw_handle* h, int32_t inst, const double* vars) {
         else ((double*)buf.data())[o] = v;
       }
   for (int b = 0; b < 2; ++b)
-    HIPCHK(hipMemcpy((char*)h->pool + (size_t)I.off[b] * h->esize, buf.data(), buf.size(),
+    HIPCHK(copy_wait(h, (char*)h->pool + (size_t)I.off[b] * h->esize, buf.data(), buf.size(),
                      hipMemcpyHostToDevice));
   return PCW_OK;
 }
@@ -762,7 +777,7 @@ int pcw_get_state(pcw_handle* h, int32_t inst, double* vars) {
   const int cur = h->hst[inst].cur;
   const int nxf = I.nx + 4, nyf = I.ny + 4;
   std::vector<char> buf((size_t)3 * I.plane * h->esize);
-  HIPCHK(hipMemcpy(buf.data(), (char*)h->pool + (size_t)I.off[cur] * h->esize, buf.size(),
+  HIPCHK(copy_wait(h, buf.data(), (char*)h->pool + (size_t)I.off[cur] * h->esize, buf.size(),
                    hipMemcpyDeviceToHost));
   for (int j = 0; j < nyf; ++j)
     for (int i = 0; i < nxf; ++i)
@@ -787,7 +802,7 @@ int pcw_set_bathymetry(pcw_handle* h, int32_t inst, const double* b) {
       if (h->esize == 4) ((float*)buf.data())[o] = (float)v;
       else ((double*)buf.data())[o] = v;
     }
-  HIPCHK(hipMemcpy((char*)h->bpool + (size_t)I.boff * h->esize, buf.data(), buf.size(),
+  HIPCHK(copy_wait(h, (char*)h->bpool + (size_t)I.boff * h->esize, buf.data(), buf.size(),
                    hipMemcpyHostToDevice));
   return PCW_OK;
 }
@@ -800,7 +815,7 @@ int pcw_set_metrics(pcw_handle* h, int32_t inst, const double* kappa, const doub
     long long off = 0;
     for (auto& I : h->hinst) off += 3 * I.plane;
     HIPCHK(hipMalloc(&h->mpool, (size_t)off * h->esize));
-    HIPCHK(hipMemset(h->mpool, 0, (size_t)off * h->esize));
+    HIPCHK(zero_wait(h, h->mpool, (size_t)off * h->esize));
     h->mpool_elems = off;
   }
   long long off = 0;
@@ -818,9 +833,9 @@ int pcw_set_metrics(pcw_handle* h, int32_t inst, const double* kappa, const doub
         if (h->esize == 4) ((float*)buf.data())[o] = (float)v;
         else ((double*)buf.data())[o] = v;
       }
-  HIPCHK(hipMemcpy((char*)h->mpool + (size_t)off * h->esize, buf.data(), buf.size(),
+  HIPCHK(copy_wait(h, (char*)h->mpool + (size_t)off * h->esize, buf.data(), buf.size(),
                    hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->dinst + inst, &I, sizeof(KInst), hipMemcpyHostToDevice));
+  HIPCHK(copy_wait(h, h->dinst + inst, &I, sizeof(KInst), hipMemcpyHostToDevice));
   h->any_met = false;
   h->any_plain = false;
   for (const auto& J : h->hinst) {
@@ -942,7 +957,7 @@ int pcw_time_init(pcw_handle* h, const pcw_time* t) {
   HIPCHK(hipMalloc(&h->dsnaps, sizeof(double) * ns));
   std::vector<double> snaps(ns);
   tc_snapshots(*t, snaps.data());
-  HIPCHK(hipMemcpy(h->dsnaps, snaps.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+  HIPCHK(copy_wait(h, h->dsnaps, snaps.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
   for (int k = 0; k < h->n; ++k) {
     InstState& S = h->hst[k];
     tc_init(S.tc, *t);
@@ -985,7 +1000,7 @@ int pcw_read_log(pcw_handle* h, int32_t inst, pcw_step_record* out, int32_t cap,
   if (rc) return rc;
   int m = std::min(cap, h->hst[inst].log_n);
   if (m > 0)
-    HIPCHK(hipMemcpy(out, h->dlog + (size_t)inst * h->log_cap, sizeof(pcw_step_record) * m,
+    HIPCHK(copy_wait(h, out, h->dlog + (size_t)inst * h->log_cap, sizeof(pcw_step_record) * m,
                      hipMemcpyDeviceToHost));
   *n = m;
   return PCW_OK;

@@ -276,3 +276,48 @@ def test_states_do_not_depend_on_tile_height(prec):
             assert np.array_equal(got[3][2:-2], ref[3][2:-2]), (rows, wv)
             assert np.array_equal(got[3][:2, 2:-2], ref[3][:2, 2:-2]), (rows, wv)
             assert np.array_equal(got[3][-2:, 2:-2], ref[3][-2:, 2:-2]), (rows, wv)
+
+
+def test_two_handles_on_two_threads_equal_one_thread():
+    """Two handles driven from two host threads at once (each thread uploads
+    its states, captures its step graphs and runs them while the other does
+    the same) end in the states of the same instances run one after the
+    other on one thread, bit for bit.  Each handle's transfers go through its
+    own non-blocking stream, so one thread's graph capture and the other's
+    copies do not meet on the legacy stream."""
+    import threading
+
+    from paraclaws_amd import abi, ic
+
+    members = [m for m in ic.ensemble_config4()[:48] if m[0] <= 400]
+    groups = [members[0::2], members[1::2]]
+    _say(f"two handles of {len(groups[0])} and {len(groups[1])} dam breaks on two threads")
+
+    def run(grp, out, slot):
+        e = _ensemble_engine(abi, ic, grp)
+        e.run(steps_per_graph=16)
+        out[slot] = [(e.status(k).accepted, e.get_state(k)) for k in range(len(grp))]
+        e.close()
+
+    alone = [None, None]
+    for g in range(2):
+        run(groups[g], alone, g)
+    both = [None, None]
+    errs = []
+
+    def guarded(g):
+        try:
+            run(groups[g], both, g)
+        except Exception as exc:  # reported below, from the main thread
+            errs.append(exc)
+
+    ths = [threading.Thread(target=guarded, args=(g,)) for g in range(2)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert not errs, errs
+    for g in range(2):
+        for (acc_a, qa), (acc_b, qb) in zip(alone[g], both[g]):
+            assert acc_a == acc_b
+            assert np.array_equal(qa.view(np.uint64), qb.view(np.uint64))
