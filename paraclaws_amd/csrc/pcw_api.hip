@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -34,7 +35,8 @@ struct pcw_handle {
   InstState* dst = nullptr;
   int4* dtiles = nullptr;
   int dtiles_cap = 0;             // tiles dtiles holds room for
-  int* dlive = nullptr;           // device copy of ntiles, read by the step kernels
+  int* dlive = nullptr;           // {tiles, first list position} of a launch, read by the
+                                  // step kernels: the whole list, lane 1, lane 2
   TileSlot* dslots = nullptr;     // FP32 completion accumulators, kSlots per instance
   int ntiles = 0;                 // tiles in the list now
   int tile_cap = 0;               // tiles the step launches' grids are sized for
@@ -55,9 +57,22 @@ struct pcw_handle {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tile_rows = 64, wpb = 2;
-  // captured step graphs by length (attempts per launch); a run uses its
-  // full length and, for the attempts left before max_accepted, a shorter one
-  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  // Lanes.  A device-controlled run of an FP32 batch splits the tile list at
+  // an instance boundary into [0, split) and [split, ntiles) and launches each
+  // half's steps on a stream of its own, so that one lane's end of launch (its
+  // last tiles finishing while others wait) overlaps the other lane's step.
+  int split = 0;                  // lane boundary in the list (0: one lane)
+  int lane_cap[2] = {0, 0};       // tiles the lanes' captured grids are sized for
+  hipStream_t stream2 = nullptr;  // lane 2 (lane 1 and the whole list: stream)
+  hipEvent_t evf = nullptr, evj = nullptr;
+  // captured step graphs by length (attempts per launch) and lane (0: the
+  // whole list); a run uses its full length and, for the attempts left
+  // before max_accepted, a shorter one
+  struct Captured {
+    int len, lane;
+    hipGraphExec_t exec;
+  };
+  std::vector<Captured> graphs;
   std::string err;
   double last_ms = 0.0, avg_step_ms = 0.0;
   long long launches = 0;
@@ -192,23 +207,24 @@ bool boundary_tile(const KInst& I, const int4& t) {
 // then start in the first round of resident waves on every XCD instead of
 // finishing last on some.  Interior tiles keep their order, and every
 // instance keeps its contiguous range [tile0, tile0+ntiles).
-void order_tiles(pcw_handle* h) {
-  const long long T = (long long)h->htiles.size();
-  if (T == 0) return;
+// lo, end: the part of the list one launch takes (a lane, or all of it).
+void order_tiles(pcw_handle* h, long long lo, long long end) {
+  const long long T = end - lo;
+  if (T <= 0) return;
   const long long nb = round_up((T + h->wpb - 1) / h->wpb, 8);
   const long long span = nb / 8 * h->wpb;
-  long long i0 = 0;
-  while (i0 < T) {
+  long long i0 = lo;
+  while (i0 < end) {
     const int k = h->htiles[i0].x;
     long long i1 = i0;
-    while (i1 < T && h->htiles[i1].x == k) ++i1;
+    while (i1 < end && h->htiles[i1].x == k) ++i1;
     const KInst& I = h->hinst[k];
     std::vector<int4> bnd, inner;
     for (long long i = i0; i < i1; ++i)
       (boundary_tile(I, h->htiles[i]) ? bnd : inner).push_back(h->htiles[i]);
     const long long n = i1 - i0, nbnd = (long long)bnd.size();
     long long out = i0, bi = 0, ii = 0;
-    for (long long a = i0 / span * span; a < i1; a += span) {
+    for (long long a = lo + (i0 - lo) / span * span; a < i1; a += span) {
       const long long hi = std::min(a + span, i1);
       // boundary tiles due up to the end of this range, in proportion
       const long long due = (hi == i1) ? nbnd : nbnd * (hi - i0) / n;
@@ -246,6 +262,23 @@ void row_bands(int ny, int ty, bool short_ends, bool short_all, std::vector<int2
   for (int b = 0; b < nb; ++b)
     out.push_back(make_int2(te + 1 + (int)((long long)mid * b / nb), te + (int)((long long)mid * (b + 1) / nb)));
   out.push_back(make_int2(ny - te + 1, ny));
+}
+
+// Lane boundary of an FP32 fused handle's list: the instance boundary nearest
+// its middle (0, one lane, when the list holds one instance).  Called while
+// every instance's tiles are contiguous and in instance order.
+void choose_split(pcw_handle* h) {
+  h->split = 0;
+  if (h->p.precision != PCW_FP32 || h->p.engine != PCW_ENGINE_FUSED || !h->stream2) return;
+  long long best = -1;
+  for (int i = 1; i < h->ntiles; ++i)
+    if (h->htiles[i].x != h->htiles[i - 1].x) {
+      const long long d = std::llabs(2LL * i - h->ntiles);
+      if (best < 0 || d < best) {
+        best = d;
+        h->split = i;
+      }
+    }
 }
 
 // Tiles of every instance: strips of kStripOut columns (the last one shifted
@@ -304,7 +337,9 @@ void build_tiles(pcw_handle* h, int ty, const std::vector<char>* only = nullptr)
     I.ntiles = (int)t.size();
   }
   h->ntiles = (int)h->htiles.size();
-  order_tiles(h);
+  choose_split(h);
+  order_tiles(h, 0, h->split > 0 ? h->split : h->ntiles);
+  if (h->split > 0) order_tiles(h, h->split, h->ntiles);
 }
 
 // tile list, descriptors and live tile count to the device (between launches)
@@ -321,7 +356,8 @@ int upload_tiles(pcw_handle* h) {
   }
   HIPCHK(copy_wait(h, h->dtiles, h->htiles.data(), sizeof(int4) * h->ntiles, hipMemcpyHostToDevice));
   HIPCHK(copy_wait(h, h->dinst, h->hinst.data(), sizeof(KInst) * h->n, hipMemcpyHostToDevice));
-  HIPCHK(copy_wait(h, h->dlive, &h->ntiles, sizeof(int), hipMemcpyHostToDevice));
+  const int lv[6] = {h->ntiles, 0, h->split, 0, h->ntiles - h->split, h->split};
+  HIPCHK(copy_wait(h, h->dlive, lv, sizeof lv, hipMemcpyHostToDevice));
   return PCW_OK;
 }
 
@@ -340,7 +376,7 @@ int push_states(pcw_handle* h) {
 }
 
 void drop_graph(pcw_handle* h) {
-  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
   h->graphs.clear();
 }
 
@@ -349,15 +385,21 @@ int limiter_code(const pcw_params& p) { return (p.higher_order && p.use_limiter)
 // enqueue one fused attempt for every instance.  Instances with metric arrays
 // and those on the identity map run in separate launches of the kernel (FP64:
 // identity-map instances whose metrics are exactly 1 in a third one).
-int enqueue_fused(pcw_handle* h) {
+// lane 0: the whole list on the handle's stream; lanes 1 and 2: the two parts
+// of a split list (FP32), each on its own stream with a grid of its own.
+int enqueue_fused(pcw_handle* h, int lane = 0) {
   const int threads = 64 * h->wpb;
-  int blocks = (h->tile_cap + h->wpb - 1) / h->wpb;
+  const int cap = lane == 0 ? h->tile_cap : h->lane_cap[lane - 1];
+  int blocks = (std::max(1, cap) + h->wpb - 1) / h->wpb;
   blocks = (int)round_up(blocks, 8);
+  const int* live = h->dlive + 2 * lane;
+  hipStream_t stream = lane == 2 ? h->stream2 : h->stream;
   if (h->p.precision == PCW_FP64) {
+    if (lane != 0) return fail(h, PCW_ERR_ARG, "lanes run FP32 lists only");
     bool has[3] = {false, false, false};
     for (const auto& I : h->hinst) has[metric_mode(I)] = true;
     FusedArgs<double> a{(double*)h->pool, (const double*)h->bpool, (const double*)h->mpool,
-                        h->dinst, h->dst, h->dtiles, h->dlive, h->tile_cap, h->p.higher_order,
+                        h->dinst, h->dst, h->dtiles, live, cap, h->p.higher_order,
                         h->p.dry_tol > 0.0, 0, h->p.grav, h->p.eps, h->p.dry_tol, nullptr};
     for (int mode = 0; mode < 3; ++mode) {
       if (!has[mode]) continue;
@@ -380,7 +422,7 @@ int enqueue_fused(pcw_handle* h) {
     any_nosrc = any_nosrc || (d.source == PCW_SRC_NONE && !wetdry);
   }
   FusedArgs<float> a{(float*)h->pool, (const float*)h->bpool, (const float*)h->mpool,
-                     h->dinst, h->dst, h->dtiles, h->dlive, h->tile_cap, h->p.higher_order,
+                     h->dinst, h->dst, h->dtiles, live, cap, h->p.higher_order,
                      wetdry, 0, h->p.grav, h->p.eps, h->p.dry_tol, h->dslots};
   for (int met = 0; met <= 1; ++met) {
     if (met ? !h->any_met : !h->any_plain) continue;
@@ -388,7 +430,7 @@ int enqueue_fused(pcw_handle* h) {
     for (int phys = 0; phys <= 1 && e == hipSuccess; ++phys) {
       if (phys ? !any_src : !any_nosrc) continue;
       e = launch_fused_f32(limiter_code(h->p), met, phys ? (wetdry ? 2 : 1) : 0, a, blocks, threads,
-                           h->stream);
+                           stream);
     }
     if (e != hipSuccess)
       return fail(h, PCW_ERR_HIP, std::string("fused launch: ") + hipGetErrorString(e));
@@ -432,9 +474,9 @@ int enqueue_source_ghosts(pcw_handle* h) {
   return PCW_OK;
 }
 
-// one device-controlled attempt (graph body)
-int enqueue_attempt(pcw_handle* h) {
-  if (h->p.engine == PCW_ENGINE_FUSED) return enqueue_fused(h);
+// one device-controlled attempt (graph body) of the whole list or of a lane
+int enqueue_attempt(pcw_handle* h, int lane) {
+  if (h->p.engine == PCW_ENGINE_FUSED) return enqueue_fused(h, lane);
   int rc = enqueue_staged_sweeps(h, 1);
   if (rc) return rc;
   rc = enqueue_source_ghosts<double>(h);
@@ -482,35 +524,37 @@ int step_bad_code(pcw_handle* h) {
   return PCW_OK;
 }
 
-// The instantiated graph of `len` device-controlled attempts, captured on
-// first use.  At most a few lengths are kept (a run uses two).
-int graph_of(pcw_handle* h, int len, hipGraphExec_t* out) {
+// The instantiated graph of `len` device-controlled attempts of the whole
+// list (lane 0) or of one lane, captured on first use on the stream it runs
+// on.  At most a few are kept (a run uses two lengths, of one or two lanes).
+int graph_of(pcw_handle* h, int len, int lane, hipGraphExec_t* out) {
   for (auto& g : h->graphs)
-    if (g.first == len) {
-      *out = g.second;
+    if (g.len == len && g.lane == lane) {
+      *out = g.exec;
       return PCW_OK;
     }
-  if (h->graphs.size() >= 4) {
-    (void)hipGraphExecDestroy(h->graphs.front().second);
+  if (h->graphs.size() >= 6) {
+    (void)hipGraphExecDestroy(h->graphs.front().exec);
     h->graphs.erase(h->graphs.begin());
   }
-  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  hipStream_t stream = lane == 2 ? h->stream2 : h->stream;
+  HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int s = 0; s < len; ++s) {
-    int rc = enqueue_attempt(h);
+    int rc = enqueue_attempt(h, lane);
     if (rc) {
       hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(h->stream, &g);
+      (void)hipStreamEndCapture(stream, &g);
       if (g) (void)hipGraphDestroy(g);
       return rc;
     }
   }
   hipGraph_t graph = nullptr;
-  HIPCHK(hipStreamEndCapture(h->stream, &graph));
+  HIPCHK(hipStreamEndCapture(stream, &graph));
   hipGraphExec_t gx = nullptr;
   hipError_t e = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(h, PCW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-  h->graphs.push_back({len, gx});
+  h->graphs.push_back({len, lane, gx});
   *out = gx;
   return PCW_OK;
 }
@@ -698,6 +742,12 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&h->ev0));
   HIPCHK(hipEventCreate(&h->ev1));
+  if (p->precision == PCW_FP32 && p->engine == PCW_ENGINE_FUSED && n_domains >= 2) {
+    // the second lane's stream (default priority, like the first)
+    HIPCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&h->evf, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&h->evj, hipEventDisableTiming));
+  }
   HIPCHK(hipMalloc(&h->pool, (size_t)base * h->esize));
   HIPCHK(zero_wait(h, h->pool, (size_t)base * h->esize));
   if (any_bathy) {
@@ -717,7 +767,7 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
     HIPCHK(hipMalloc(&h->work, (size_t)welems * sizeof(double)));
     HIPCHK(zero_wait(h, h->work, (size_t)welems * sizeof(double)));
   }
-  HIPCHK(hipMalloc(&h->dlive, sizeof(int)));
+  HIPCHK(hipMalloc(&h->dlive, 6 * sizeof(int)));
   build_tiles(h, h->tile_rows);
   h->tile_cap = h->ntiles;
   int rc = upload_tiles(h);
@@ -729,12 +779,16 @@ int pcw_destroy(pcw_handle* h) {
   if (!h) return PCW_ERR_ARG;
   drop_graph(h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
   void* ptrs[] = {h->pool, h->bpool, h->mpool, h->dinst, h->dst, h->dtiles, h->dslots, h->dsnaps, h->dlog,
                   h->work, h->dscratch, h->dlive};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->evf) (void)hipEventDestroy(h->evf);
+  if (h->evj) (void)hipEventDestroy(h->evj);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return PCW_OK;
@@ -1057,14 +1111,46 @@ int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts, int64_
       len = (int)std::max(1ll, std::min<long long>(len, need));
     }
     if (max_attempts > 0) len = (int)std::max(1ll, std::min<long long>(len, max_attempts - attempts));
-    hipGraphExec_t gx = nullptr;
-    rc = graph_of(h, len, &gx);
-    if (rc) return rc;
+    // two lanes while both halves of the list hold running instances
+    bool run_lo = false, run_hi = false;
+    for (int k = 0; k < h->n; ++k)
+      if (h->hst[k].status == PCW_STATUS_RUNNING && h->hinst[k].ntiles > 0)
+        (h->hinst[k].tile0 < h->split ? run_lo : run_hi) = true;
+    const bool lanes = h->split > 0 && run_lo && run_hi;
     long long a0 = 0;
     for (int k = 0; k < h->n; ++k) a0 = std::max(a0, h->hst[k].accepted + h->hst[k].rejected);
-    HIPCHK(hipEventRecord(h->ev0, h->stream));
-    HIPCHK(hipGraphLaunch(gx, h->stream));
-    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    if (lanes) {
+      // lane grids are captured for the lanes' sizes; a lane that outgrew
+      // its grid or shrank below a quarter of it is captured again
+      const int want[2] = {h->split, h->ntiles - h->split};
+      bool fits = true;
+      for (int l = 0; l < 2; ++l)
+        fits = fits && want[l] <= h->lane_cap[l] && (long long)want[l] * 4 >= h->lane_cap[l];
+      if (!fits) {
+        drop_graph(h);
+        h->lane_cap[0] = want[0];
+        h->lane_cap[1] = want[1];
+      }
+      hipGraphExec_t g1 = nullptr, g2 = nullptr;
+      rc = graph_of(h, len, 1, &g1);
+      if (!rc) rc = graph_of(h, len, 2, &g2);
+      if (rc) return rc;
+      HIPCHK(hipEventRecord(h->ev0, h->stream));
+      HIPCHK(hipEventRecord(h->evf, h->stream));  // lane 2 starts after the uploads on stream
+      HIPCHK(hipStreamWaitEvent(h->stream2, h->evf, 0));
+      HIPCHK(hipGraphLaunch(g1, h->stream));
+      HIPCHK(hipGraphLaunch(g2, h->stream2));
+      HIPCHK(hipEventRecord(h->evj, h->stream2));  // and stream waits for lane 2
+      HIPCHK(hipStreamWaitEvent(h->stream, h->evj, 0));
+      HIPCHK(hipEventRecord(h->ev1, h->stream));
+    } else {
+      hipGraphExec_t gx = nullptr;
+      rc = graph_of(h, len, 0, &gx);
+      if (rc) return rc;
+      HIPCHK(hipEventRecord(h->ev0, h->stream));
+      HIPCHK(hipGraphLaunch(gx, h->stream));
+      HIPCHK(hipEventRecord(h->ev1, h->stream));
+    }
     rc = pull_states(h);
     if (rc) return rc;
     float ms = 0.f;

@@ -566,12 +566,15 @@ k_step(FusedArgs<float> a) {
   // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous
   // range of the live tiles so neighbouring strips share that XCD's L2.
   // Blocks beyond the live list (the grid was sized for a longer one) exit.
-  const int nlive = __builtin_amdgcn_readfirstlane(*a.live);
+  // The launch takes the nlive tiles from list position base on.
+  const int nlive = __builtin_amdgcn_readfirstlane(a.live[0]);
+  const int base = __builtin_amdgcn_readfirstlane(a.live[1]);
   const int nb = ((nlive + wpb - 1) / wpb + 7) & ~7, per = nb >> 3;
   if ((int)blockIdx.x >= nb) return;
   const int bb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  const int tid = __builtin_amdgcn_readfirstlane(bb * wpb + (threadIdx.x >> 6));
-  if (tid >= nlive) return;
+  const int lt = __builtin_amdgcn_readfirstlane(bb * wpb + (threadIdx.x >> 6));
+  if (lt >= nlive) return;
+  const int tid = base + lt;  // position in the tile list
   const int4 t = a.tiles[tid];
   const int in = __builtin_amdgcn_readfirstlane(t.x);
   const KInst& I = a.inst[in];

@@ -39,8 +39,10 @@ struct FusedArgs {
   const KInst* inst;
   InstState* st;
   const int4* tiles;   // (instance, x0, y0, y1): output columns x0..x0+59, rows y0..y1
-  const int* live;     // number of tiles in the list now (device memory: the host may shrink
-                       // the list between launches of a captured graph)
+  const int* live;     // live[0]: tiles this launch takes from the list, live[1]: the list
+                       // position of the first (device memory: the host may change both
+                       // between launches of a captured graph; a batch may run as two
+                       // launches over the two halves of one list)
   int ntiles;          // tiles the launch's grid was sized for
   int ho;              // add_higher_order_flux_correction
   int wetdry;          // dry_tol > 0

@@ -26,7 +26,7 @@ namespace pcw {
 namespace {
 
 // scratch layout (bytes): descriptor, controller record, kSlots completion
-// accumulators, tile count, tile list
+// accumulators, tile count and first list position, tile list
 constexpr size_t kScratchBytes = size_t(2) << 20;
 constexpr size_t kOffInst = 0, kOffState = 512, kOffSlots = 1024, kOffLive = 6144, kOffTiles = 8192;
 constexpr long long kMaxTiles = (long long)((kScratchBytes - kOffTiles) / sizeof(int4));
@@ -68,7 +68,8 @@ __global__ void k_noalloc_setup(unsigned char* scr, KInst I, double dt, int nsx,
     S.last_ok = 1;
     S.tc.dt = dt;
     *reinterpret_cast<InstState*>(scr + kOffState) = S;
-    *reinterpret_cast<int*>(scr + kOffLive) = I.ntiles;
+    reinterpret_cast<int*>(scr + kOffLive)[0] = I.ntiles;  // tiles of the launch
+    reinterpret_cast<int*>(scr + kOffLive)[1] = 0;         // from list position 0
   }
   constexpr int kSlotWords = (int)(sizeof(TileSlot) * kSlots / sizeof(unsigned));
   if (t < kSlotWords) reinterpret_cast<unsigned*>(scr + kOffSlots)[t] = 0u;
