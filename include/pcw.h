@@ -210,7 +210,9 @@ int pcw_read_log(pcw_handle* h, int32_t inst, pcw_step_record* out, int32_t cap,
  * a block is then shortened to the accepted steps the running instances
  * still need, so that no launch of it only finds finished instances.  An
  * instance that an earlier run stopped at its max_accepted continues when
- * the limit is raised (or 0). */
+ * the limit is raised (or 0).  An FP32 batch runs as two lanes, halves of
+ * its instances whose step graphs run on two streams at once; the call
+ * returns after both. */
 int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts,
             int64_t max_accepted);
 /* on = 1: device-controlled runs stop each instance after the accepted step
