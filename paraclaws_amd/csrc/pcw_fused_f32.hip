@@ -132,7 +132,8 @@ struct Ctx {
   int nslot;            // interior slots of a frame row (slot rows)
   int bc[4];
   float gr[8];       // extrapolation ratios of the linear-extrapolation boundary
-  float dtdx, dtdy, eps, g, dry, n2, gdx, gdy, dt;
+  float dtdx, dtdy, eps, g, dry;
+  float sx, sy, fk;  // source-term constants of the step (see source)
   bool out_lane, xface, xlim;
 };
 __device__ __forceinline__ float lft(const Ctx& C, float v) { return bperm(C.bl, v); }
@@ -250,10 +251,12 @@ __device__ __forceinline__ void corr_flux(float coef, float p1, float p2, const 
 }
 
 // source term of one finished cell (wet/dry cut-off, bathymetry slope,
-// semi-implicit Manning friction), FP32 with reciprocals
+// semi-implicit Manning friction), FP32 with reciprocals.  The instance's
+// constants come folded: sx = -dt g / dxi, sy = -dt g / deta (slope, times
+// the central difference bx = b(i+1) - b(i-1), by = b(j+1) - b(j-1)), and
+// fk = dt g n^2 (friction: hu / (1 + fk |u| h^(-4/3))).
 __device__ __forceinline__ void source(float& h, float& hu, float& hv, int kind, float dry,
-                                       float g, float gdx, float gdy, float dt, float n2, float bx,
-                                       float by) {
+                                       float sx, float sy, float fk, float bx, float by) {
   if (!(h > dry)) {
     h = h > 0.0f ? h : 0.0f;
     hu = 0.0f;
@@ -262,15 +265,15 @@ __device__ __forceinline__ void source(float& h, float& hu, float& hv, int kind,
   }
   float nhu = hu, nhv = hv;
   if (kind == PCW_SRC_BATHY || kind == PCW_SRC_BATHY_FRICTION) {
-    nhu = fmaf(dt, -gdx * h * bx, hu);
-    nhv = fmaf(dt, -gdy * h * by, hv);
+    nhu = fmaf(sx * bx, h, hu);
+    nhv = fmaf(sy * by, h, hv);
   }
   if (kind == PCW_SRC_FRICTION || kind == PCW_SRC_BATHY_FRICTION) {
     const float rh = rcp(h);
     const float u = hu * rh, v = hv * rh;
     const float sp = vsqrt(fmaf(u, u, v * v));
-    const float h43 = __builtin_amdgcn_exp2f(1.33333333f * __builtin_amdgcn_logf(h));
-    const float rden = rcp(fmaf(dt * g * n2, sp * rcp(h43), 1.0f));
+    const float hm43 = __builtin_amdgcn_exp2f(-1.33333333f * __builtin_amdgcn_logf(h));  // h^(-4/3)
+    const float rden = rcp(fmaf(fk, sp * hm43, 1.0f));
     nhu = nhu * rden;
     nhv = nhv * rden;
   }
@@ -489,8 +492,10 @@ __device__ __forceinline__ void finish(const Ctx& C, int r, const Slot& N, const
 #pragma unroll
     for (int k = 0; k < 3; ++k) qn[k] = PS ? acc[k] - dH[k] : fmaf(-facB, dH[k], acc[k]);
     if (PHYS == 2 || (PHYS == 1 && C.src != PCW_SRC_NONE)) {
-      const float bx = shl1(b0) - shr1(b0), by = bnew - bm;
-      source(qn[0], qn[1], qn[2], C.src, C.dry, C.g, C.gdx, C.gdy, C.dt, C.n2, bx, by);
+      // b(i+1) - b(i-1) through the LDS crossbar (lanes 0 and 63, which
+      // receive each other's values, are halo lanes)
+      const float bx = rgt(C, b0) - lft(C, b0), by = bnew - bm;
+      source(qn[0], qn[1], qn[2], C.src, C.dry, C.sx, C.sy, C.fk, bx, by);
     }
     const int sj = (j + 1 - C.rbase) * C.p4;
     // an interior tile starts in the unconditional body: its first rows
@@ -640,10 +645,9 @@ k_step(FusedArgs<float> a) {
   C.eps = (float)a.eps;
   C.g = (float)a.g;
   C.dry = (float)a.dry;
-  C.n2 = (float)(I.manning * I.manning);
-  C.gdx = (float)(a.g / I.dxi);
-  C.gdy = (float)(a.g / I.deta);
-  C.dt = (float)dt;
+  C.sx = (float)(-dt * a.g / I.dxi);
+  C.sy = (float)(-dt * a.g / I.deta);
+  C.fk = (float)(dt * a.g * I.manning * I.manning);
 
   Red R;
   R.smx = R.smy = 0.0f;
