@@ -321,3 +321,59 @@ def test_two_handles_on_two_threads_equal_one_thread():
         for (acc_a, qa), (acc_b, qb) in zip(alone[g], both[g]):
             assert acc_a == acc_b
             assert np.array_equal(qa.view(np.uint64), qb.view(np.uint64))
+
+
+def test_fp32_batch_in_pieces_equals_one_run():
+    """An FP32 batch runs in two lanes (two halves of its instances, their
+    step graphs on two streams at once), its tile list rebuilt and split
+    again as instances finish.  Stopping it at accepted-step limits, retuning
+    its tiles in between (the autotuner times both lanes at once) and going
+    on with other graph lengths ends, bit for bit, where one uninterrupted run
+    ends; two members run alone end there too."""
+    from paraclaws_amd import abi, ic
+
+    sizes = [(40, 40, 2.0), (70, 50, 3.5), (120, 90, 1.7), (33, 150, 6.0), (200, 64, 2.5),
+             (90, 90, 4.0)]
+    kw = dict(t_final=0.05, dt0=1e-5, cfl_target=0.9, dt_max=1e-2, dt_min=1e-10, growth=1.2,
+              shrink=0.5, n_snapshots=2)
+
+    def engine(sel):
+        doms = [abi.Domain(sizes[k][0], sizes[k][1], bc=(4, 3, 2, 4)) for k in sel]
+        e = abi.Engine(doms, precision=abi.FP32, limiter=4, tile_rows=16, waves_per_block=2)
+        for j, k in enumerate(sel):
+            nx, ny, r = sizes[k]
+            e.set_state(j, ic.dam_break(nx, ny, h_in=r))
+        e.fill_ghosts()
+        e.time_init(**kw)
+        return e
+
+    def result(e, n):
+        out = []
+        for j in range(n):
+            st = e.status(j)
+            assert st.status == abi.STATUS_DONE, (j, st.status)
+            out.append(((st.accepted, st.rejected, st.time), e.get_state(j)))
+        return out
+
+    everyone = list(range(len(sizes)))
+    _say("FP32 batch of six, one run")
+    with engine(everyone) as e:
+        e.run(steps_per_graph=8)
+        whole = result(e, len(sizes))
+    _say("FP32 batch of six, in pieces with a retune")
+    with engine(everyone) as e:
+        e.run(steps_per_graph=8, max_accepted=10)
+        e.autotune(steps=2)
+        e.run(steps_per_graph=5, max_accepted=25)
+        e.run(steps_per_graph=8)
+        pieces = result(e, len(sizes))
+    for k in everyone:
+        assert whole[k][0] == pieces[k][0], k
+        assert np.array_equal(whole[k][1].view(np.uint64), pieces[k][1].view(np.uint64)), k
+    for k in (0, 4):
+        _say(f"member {k} alone")
+        with engine([k]) as e:
+            e.run(steps_per_graph=8)
+            (st, q), = result(e, 1)
+        assert st == whole[k][0]
+        assert np.array_equal(q.view(np.uint64), whole[k][1].view(np.uint64)), k
