@@ -241,6 +241,21 @@ int pcw_kernel_stats(pcw_handle* h, double* avg_step_ms, int64_t* launches);
  * median launch (bytes moved = 6 * plane_bytes).  Allocates and frees its
  * six planes. */
 int pcw_stream_bandwidth(int64_t plane_bytes, int32_t reps, double* gbs_best, double* gbs_median);
+
+/* The tile list pcw_create would step these instances with, for a given
+ * band height (p->tile_rows; 0 means 64 rows in FP32, 128 in FP64, without
+ * the shortening pcw_create applies where tiles would not fill the GPU) and
+ * p->waves_per_block, computed on the host: it touches no device.  Tile t is
+ * out[4t .. 4t+3] = (instance, x0, y0, y1): output columns x0 .. x0+59
+ * (within 1 .. nx; the last strip is shifted to end at column nx), rows
+ * y0 .. y1, each instance's tiles contiguous.  lanes != 0: the list of an
+ * FP32 fused batch is split for pcw_run's two lanes, *split receiving the
+ * list position where the second starts (0: one lane).  Writes at most cap
+ * tiles (out may be NULL when cap is 0) and returns the number of tiles, or
+ * PCW_ERR_ARG.  It has no counterpart in the reference: it exposes the
+ * engine's scheduling to tests and tools. */
+int64_t pcw_plan_tiles(const pcw_params* p, const pcw_domain* d, int32_t n, int32_t lanes,
+                       int32_t* out, int64_t cap, int32_t* split);
 /* Autotuner: times candidate (tile_rows, waves_per_block) on the handle's
  * current state and keeps the fastest; writes the choice. */
 int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows,

@@ -94,6 +94,8 @@ SIGNATURES = {
     "pcw_kernel_stats": ([_h, _dp, _P(C.c_int64)], C.c_int),
     "pcw_autotune": ([_h, C.c_int32, _P(C.c_int32), _P(C.c_int32)], C.c_int),
     "pcw_stream_bandwidth": ([C.c_int64, C.c_int32, _dp, _dp], C.c_int),
+    "pcw_plan_tiles": ([_P(PcwParams), _P(PcwDomain), C.c_int32, C.c_int32, _P(C.c_int32), C.c_int64,
+                        _P(C.c_int32)], C.c_int64),
     "pcw_step_noalloc": ([_P(PcwParams), _P(PcwDomain), C.c_void_p, C.c_void_p, C.c_double, _dp,
                           C.c_void_p], C.c_int),
 }
@@ -184,6 +186,24 @@ def stream_bandwidth(plane_bytes: int, reps: int = 10):
     if rc != PCW_OK:
         raise PcwError(rc, "pcw_stream_bandwidth failed")
     return best.value, med.value
+
+
+def plan_tiles(params, domains, lanes=False):
+    """pcw_plan_tiles: the tile list a handle of `domains` steps with, computed
+    on the host (no device).  Returns (tiles, split): an (n, 4) int32 array of
+    (instance, x0, y0, y1) and the list position of the second lane (0: one)."""
+    L = load()
+    n = len(domains)
+    arr = (PcwDomain * n)(*[d.c() for d in domains])
+    split = C.c_int32(0)
+    total = L.pcw_plan_tiles(C.byref(params), arr, n, int(bool(lanes)), None, 0, C.byref(split))
+    if total < 0:
+        raise PcwError(int(total), "pcw_plan_tiles: bad argument")
+    out = np.zeros((max(total, 1), 4), dtype=np.int32)
+    got = L.pcw_plan_tiles(C.byref(params), arr, n, int(bool(lanes)),
+                           out.ctypes.data_as(_P(C.c_int32)), total, C.byref(split))
+    assert got == total
+    return out[:total], split.value
 
 
 def make_params(precision=FP64, engine=FUSED, higher_order=True, use_limiter=True, limiter=4,

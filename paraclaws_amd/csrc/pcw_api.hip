@@ -62,6 +62,7 @@ struct pcw_handle {
   // half's steps on a stream of its own, so that one lane's end of launch (its
   // last tiles finishing while others wait) overlaps the other lane's step.
   int split = 0;                  // lane boundary in the list (0: one lane)
+  bool two_lanes = false;         // an FP32 fused handle of two or more instances
   int lane_cap[2] = {0, 0};       // tiles the lanes' captured grids are sized for
   hipStream_t stream2 = nullptr;  // lane 2 (lane 1 and the whole list: stream)
   hipEvent_t evf = nullptr, evj = nullptr;
@@ -269,7 +270,7 @@ void row_bands(int ny, int ty, bool short_ends, bool short_all, std::vector<int2
 // every instance's tiles are contiguous and in instance order.
 void choose_split(pcw_handle* h) {
   h->split = 0;
-  if (h->p.precision != PCW_FP32 || h->p.engine != PCW_ENGINE_FUSED || !h->stream2) return;
+  if (!h->two_lanes) return;
   long long best = -1;
   for (int i = 1; i < h->ntiles; ++i)
     if (h->htiles[i].x != h->htiles[i - 1].x) {
@@ -753,6 +754,7 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
   if (p->precision == PCW_FP32 && p->engine == PCW_ENGINE_FUSED && n_domains >= 2) {
     // the second lane's stream (default priority, like the first)
     HIPCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    h->two_lanes = true;
     HIPCHK(hipEventCreateWithFlags(&h->evf, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&h->evj, hipEventDisableTiming));
   }
@@ -1229,6 +1231,34 @@ int pcw_last_elapsed_ms(pcw_handle* h, double* ms) {
   if (!h || !ms) return PCW_ERR_ARG;
   *ms = h->last_ms;
   return PCW_OK;
+}
+
+int64_t pcw_plan_tiles(const pcw_params* p, const pcw_domain* d, int32_t n, int32_t lanes,
+                       int32_t* out, int64_t cap, int32_t* split) {
+  if (!p || !d || n < 1 || n > PCW_MAX_DOMAINS || cap < 0 || (cap > 0 && !out)) return PCW_ERR_ARG;
+  if (p->precision != PCW_FP32 && p->precision != PCW_FP64) return PCW_ERR_ARG;
+  for (int k = 0; k < n; ++k)
+    if (d[k].nx < 2 || d[k].ny < 2 || d[k].nx > PCW_MAX_SIDE || d[k].ny > PCW_MAX_SIDE)
+      return PCW_ERR_ARG;
+  pcw_handle h;  // host side only: no stream, no device memory
+  h.p = *p;
+  h.n = n;
+  h.esize = p->precision == PCW_FP32 ? 4 : 8;
+  h.dom.assign(d, d + n);
+  h.hinst.resize(n);
+  for (int k = 0; k < n; ++k) describe_instance(*p, d[k], h.hinst[k]);
+  h.wpb = (p->waves_per_block >= 1 && p->waves_per_block <= 4) ? p->waves_per_block : 2;
+  h.two_lanes = lanes != 0 && p->precision == PCW_FP32 && p->engine == PCW_ENGINE_FUSED && n >= 2;
+  build_tiles(&h, p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128));
+  if (split) *split = h.split;
+  for (long long t = 0; t < std::min<long long>(cap, h.ntiles); ++t) {
+    const int4 v = h.htiles[t];
+    out[4 * t] = v.x;
+    out[4 * t + 1] = v.y;
+    out[4 * t + 2] = v.z;
+    out[4 * t + 3] = v.w;
+  }
+  return h.ntiles;
 }
 
 int pcw_kernel_stats(pcw_handle* h, double* avg_step_ms, int64_t* launches) {

@@ -18,7 +18,7 @@ HEADER = os.path.join(ROOT, "include", "pcw.h")
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(pcw_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(pcw_\w+)\s*\(", text, re.M)))
 
 
 def test_library_loads_and_exports_every_declared_symbol():
