@@ -1320,6 +1320,11 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
   cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
   double best = 1e30;
   int br = h->tile_rows, bw = h->wpb;
+  struct Timed {
+    float ms;
+    int rows, waves;
+  };
+  std::vector<Timed> timed;
   for (const auto& c : cand) {
     const int tr = c.first, wv = c.second;
     h->wpb = wv;
@@ -1353,11 +1358,31 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
     HIPCHK(hipEventSynchronize(h->ev1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    timed.push_back({ms, tr, wv});
     if (ms < best) {
       best = ms;
       br = tr;
       bw = wv;
     }
+  }
+  if (h->split > 0) {
+    // A batch: the candidates are timed at the run's current state, where
+    // small instances can make short and tall bands time alike; the large
+    // instances that run on after them take fewer halo rows with tall ones
+    // (the 512-instance ensemble: 0.412 s with 64-row tiles against 0.417 s
+    // with 32-row ones, which timed alike at its start).  So the tallest
+    // of the candidates within 1 % of the fastest gives the height, and the
+    // fastest of that height is taken.
+    int tallest = br;
+    for (const auto& c : timed)
+      if (c.ms <= 1.01 * best) tallest = std::max(tallest, c.rows);
+    float fastest = 1e30f;
+    for (const auto& c : timed)
+      if (c.rows == tallest && c.ms < fastest) {
+        fastest = c.ms;
+        br = c.rows;
+        bw = c.waves;
+      }
   }
   h->tile_rows = br;
   h->wpb = bw;
