@@ -1233,6 +1233,8 @@ int pcw_last_elapsed_ms(pcw_handle* h, double* ms) {
   return PCW_OK;
 }
 
+constexpr long long kMaxPlannedTiles = 1LL << 28;
+
 int64_t pcw_plan_tiles(const pcw_params* p, const pcw_domain* d, int32_t n, int32_t lanes,
                        int32_t* out, int64_t cap, int32_t* split) {
   if (!p || !d || n < 1 || n > PCW_MAX_DOMAINS || cap < 0 || (cap > 0 && !out)) return PCW_ERR_ARG;
@@ -1249,7 +1251,9 @@ int64_t pcw_plan_tiles(const pcw_params* p, const pcw_domain* d, int32_t n, int3
   for (int k = 0; k < n; ++k) describe_instance(*p, d[k], h.hinst[k]);
   h.wpb = (p->waves_per_block >= 1 && p->waves_per_block <= 4) ? p->waves_per_block : 2;
   h.two_lanes = lanes != 0 && p->precision == PCW_FP32 && p->engine == PCW_ENGINE_FUSED && n >= 2;
-  build_tiles(&h, p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128));
+  const int ty = p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128);
+  if (count_tiles(&h, ty) > kMaxPlannedTiles) return PCW_ERR_ARG;  // list positions are ints
+  build_tiles(&h, ty);
   if (split) *split = h.split;
   for (long long t = 0; t < std::min<long long>(cap, h.ntiles); ++t) {
     const int4 v = h.htiles[t];

@@ -133,7 +133,7 @@ def test_fp32_edge_tiles_are_shorter():
     assert rows[inner].max() <= ty and rows[inner].min() > te
 
 
-@pytest.mark.parametrize("bad", ["nx", "ny", "count", "precision", "cap"])
+@pytest.mark.parametrize("bad", ["nx", "ny", "count", "precision", "cap", "huge"])
 def test_plan_rejects_bad_arguments(bad):
     import ctypes as C
 
@@ -152,6 +152,13 @@ def test_plan_rejects_bad_arguments(bad):
         p.precision = 7
     elif bad == "cap":
         cap = 10  # no output array
+    elif bad == "huge":  # more tiles than list positions allow
+        p.tile_rows = 3
+        n = 2000
+        d = (abi.PcwDomain * n)(*[abi.Domain(200000, 200000).c() for _ in range(n)])
+        rc = L.pcw_plan_tiles(C.byref(p), d, n, 0, out, cap, None)
+        assert rc == abi.PCW_ERR_ARG
+        return
     rc = L.pcw_plan_tiles(C.byref(p), C.byref(d), n, 0, out, cap, None)
     assert rc == abi.PCW_ERR_ARG
 
