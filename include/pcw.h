@@ -210,7 +210,7 @@ int pcw_read_log(pcw_handle* h, int32_t inst, pcw_step_record* out, int32_t cap,
  * a block is then shortened to the accepted steps the running instances
  * still need, so that no launch of it only finds finished instances.  An
  * instance that an earlier run stopped at its max_accepted continues when
- * the limit is raised (or 0).  An FP32 batch runs as two lanes, halves of
+ * the limit is raised (or 0).  A fused batch runs as two lanes, halves of
  * its instances whose step graphs run on two streams at once; the call
  * returns after both. */
 int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts,
@@ -248,8 +248,8 @@ int pcw_stream_bandwidth(int64_t plane_bytes, int32_t reps, double* gbs_best, do
  * p->waves_per_block, computed on the host: it touches no device.  Tile t is
  * out[4t .. 4t+3] = (instance, x0, y0, y1): output columns x0 .. x0+59
  * (within 1 .. nx; the last strip is shifted to end at column nx), rows
- * y0 .. y1, each instance's tiles contiguous.  lanes != 0: the list of an
- * FP32 fused batch is split for pcw_run's two lanes, *split receiving the
+ * y0 .. y1, each instance's tiles contiguous.  lanes != 0: the list of a
+ * fused batch is split for pcw_run's two lanes, *split receiving the
  * list position where the second starts (0: one lane).  Writes at most cap
  * tiles (out may be NULL when cap is 0) and returns the number of tiles, or
  * PCW_ERR_ARG.  It has no counterpart in the reference: it exposes the

@@ -57,12 +57,13 @@ struct pcw_handle {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tile_rows = 64, wpb = 2;
-  // Lanes.  A device-controlled run of an FP32 batch splits the tile list at
+  // Lanes.  A device-controlled run of a fused batch splits the tile list at
   // an instance boundary into [0, split) and [split, ntiles) and launches each
   // half's steps on a stream of its own, so that one lane's end of launch (its
   // last tiles finishing while others wait) overlaps the other lane's step.
   int split = 0;                  // lane boundary in the list (0: one lane)
-  bool two_lanes = false;         // an FP32 fused handle of two or more instances
+  bool two_lanes = false;         // a fused handle of two or more instances
+  int lane_k = 0;                 // first instance of lane 2 in the captured lane graphs
   int lane_cap[2] = {0, 0};       // tiles the lanes' captured grids are sized for
   hipStream_t stream2 = nullptr;  // lane 2 (lane 1 and the whole list: stream)
   hipEvent_t evf = nullptr, evj = nullptr;
@@ -265,7 +266,7 @@ void row_bands(int ny, int ty, bool short_ends, bool short_all, std::vector<int2
   out.push_back(make_int2(ny - te + 1, ny));
 }
 
-// Lane boundary of an FP32 fused handle's list: the instance boundary nearest
+// Lane boundary of a fused handle's list: the instance boundary nearest
 // its middle (0, one lane, when the list holds one instance).  Called while
 // every instance's tiles are contiguous and in instance order.
 void choose_split(pcw_handle* h) {
@@ -396,7 +397,6 @@ int enqueue_fused(pcw_handle* h, int lane = 0) {
   const int* live = h->dlive + 2 * lane;
   hipStream_t stream = lane == 2 ? h->stream2 : h->stream;
   if (h->p.precision == PCW_FP64) {
-    if (lane != 0) return fail(h, PCW_ERR_ARG, "lanes run FP32 lists only");
     bool has[3] = {false, false, false};
     for (const auto& I : h->hinst) has[metric_mode(I)] = true;
     FusedArgs<double> a{(double*)h->pool, (const double*)h->bpool, (const double*)h->mpool,
@@ -404,13 +404,16 @@ int enqueue_fused(pcw_handle* h, int lane = 0) {
                         h->p.dry_tol > 0.0, 0, h->p.grav, h->p.eps, h->p.dry_tol, nullptr};
     for (int mode = 0; mode < 3; ++mode) {
       if (!has[mode]) continue;
-      hipError_t e = launch_fused_f64(limiter_code(h->p), mode, a, blocks, threads, h->stream);
+      hipError_t e = launch_fused_f64(limiter_code(h->p), mode, a, blocks, threads, stream);
       if (e != hipSuccess)
         return fail(h, PCW_ERR_HIP, std::string("fused launch: ") + hipGetErrorString(e));
     }
-    // ghost cells of the new state
-    hipError_t e = launch_ghosts<double>((double*)h->pool, h->dinst, h->dst, h->n, h->max_side,
-                                         kGhostsAfterFused, h->stream);
+    // ghost cells of the new state, of the lane's instances: lane 1 holds the
+    // instances below lane_k, lane 2 the others (a captured lane graph is
+    // captured again when the split moves to another instance)
+    const int k0 = lane == 2 ? h->lane_k : 0, k1 = lane == 1 ? h->lane_k : h->n;
+    hipError_t e = launch_ghosts<double>((double*)h->pool, h->dinst + k0, h->dst + k0, k1 - k0,
+                                         h->max_side, kGhostsAfterFused, stream);
     if (e != hipSuccess)
       return fail(h, PCW_ERR_HIP, std::string("ghost launch: ") + hipGetErrorString(e));
     return PCW_OK;
@@ -751,7 +754,7 @@ int pcw_create(const pcw_params* p, const pcw_domain* domains, int32_t n_domains
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&h->ev0));
   HIPCHK(hipEventCreate(&h->ev1));
-  if (p->precision == PCW_FP32 && p->engine == PCW_ENGINE_FUSED && n_domains >= 2) {
+  if (p->engine == PCW_ENGINE_FUSED && n_domains >= 2) {
     // the second lane's stream (default priority, like the first)
     HIPCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
     h->two_lanes = true;
@@ -1133,13 +1136,17 @@ int pcw_run(pcw_handle* h, int32_t steps_per_graph, int64_t max_attempts, int64_
       // lane grids are captured for the lanes' sizes; a lane that outgrew
       // its grid or shrank below a quarter of it is captured again
       const int want[2] = {h->split, h->ntiles - h->split};
+      const int k_split = h->htiles[h->split].x;
       bool fits = true;
       for (int l = 0; l < 2; ++l)
         fits = fits && want[l] <= h->lane_cap[l] && (long long)want[l] * 4 >= h->lane_cap[l];
+      // the FP64 ghost launches of a lane graph name its instances
+      if (h->p.precision == PCW_FP64) fits = fits && k_split == h->lane_k;
       if (!fits) {
         drop_graph(h);
         h->lane_cap[0] = want[0];
         h->lane_cap[1] = want[1];
+        h->lane_k = k_split;
       }
       hipGraphExec_t g1 = nullptr, g2 = nullptr;
       rc = graph_of(h, len, 1, &g1);
@@ -1250,7 +1257,7 @@ int64_t pcw_plan_tiles(const pcw_params* p, const pcw_domain* d, int32_t n, int3
   h.hinst.resize(n);
   for (int k = 0; k < n; ++k) describe_instance(*p, d[k], h.hinst[k]);
   h.wpb = (p->waves_per_block >= 1 && p->waves_per_block <= 4) ? p->waves_per_block : 2;
-  h.two_lanes = lanes != 0 && p->precision == PCW_FP32 && p->engine == PCW_ENGINE_FUSED && n >= 2;
+  h.two_lanes = lanes != 0 && p->engine == PCW_ENGINE_FUSED && n >= 2;
   const int ty = p->tile_rows > 1 ? p->tile_rows : (p->precision == PCW_FP32 ? 64 : 128);
   if (count_tiles(&h, ty) > kMaxPlannedTiles) return PCW_ERR_ARG;  // list positions are ints
   build_tiles(&h, ty);
@@ -1337,11 +1344,12 @@ int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows, int32_t* wave
     rc = upload_tiles(h);
     if (rc) return rc;
     // one warm-up attempt, then `steps` timed ones, launched as pcw_run
-    // launches them: the whole list on one stream, or an FP32 batch's two
-    // lanes at once on their two streams
+    // launches them: the whole list on one stream, or a batch's two lanes
+    // at once on their two streams
     const bool lanes = h->split > 0;
     h->lane_cap[0] = h->split;
     h->lane_cap[1] = h->ntiles - h->split;
+    h->lane_k = lanes ? h->htiles[h->split].x : h->n;
     for (int pass = 0; pass < 2; ++pass) {
       if (pass == 1) HIPCHK(hipEventRecord(h->ev0, h->stream));
       if (lanes) {

@@ -6,9 +6,9 @@ cell of every instance must be produced, each row of a strip exactly once,
 by bands of at least two rows (the FP32 step finishes the eta ghost rows from
 the two rows next to the boundary inside one tile).  Each instance's tiles are
 contiguous in the list, because the completion of an attempt counts the tiles
-from the instance's first one.  An FP32 batch's list is split into two lanes
-at the instance boundary nearest its middle; FP64 lists and single instances
-are not split.  FP32 tiles span fewer than 2^31 bytes of a plane, the range of
+from the instance's first one.  A fused batch's list is split into two lanes
+at the instance boundary nearest its middle; single instances and the staged
+engine's lists are not split.  FP32 tiles span fewer than 2^31 bytes of a plane, the range of
 the buffer offsets the step kernel addresses them with.
 """
 from __future__ import annotations
@@ -66,8 +66,7 @@ def test_tiles_cover_every_cell_once_per_strip(prec, tile_rows, wpb):
         assert mine.size and (np.diff(mine) == 1).all(), k
         assert mine[0] == starts[k]
         _check_instance(tiles[mine], nx, ny)
-    if precision == abi.FP64:
-        assert split == 0
+    assert 0 < split < len(tiles) and tiles[split - 1, 0] != tiles[split, 0]
 
 
 @pytest.mark.parametrize("n_inst", [1, 2, 3, 7, 40])
@@ -85,10 +84,15 @@ def test_lanes_split_at_the_instance_boundary_nearest_the_middle(n_inst):
     bounds = [i for i in range(1, T) if tiles[i, 0] != tiles[i - 1, 0]]
     assert split in bounds
     assert abs(2 * split - T) == min(abs(2 * b - T) for b in bounds)
-    # one lane when the caller does not ask for lanes, and never in FP64
+    # one lane when the caller does not ask for lanes, and for the staged engine
     assert abi.plan_tiles(p, doms, lanes=False)[1] == 0
+    st = abi.make_params(abi.FP64, engine=abi.STAGED, limiter=4, tile_rows=32, waves_per_block=4)
+    assert abi.plan_tiles(st, doms, lanes=True)[1] == 0
+    # FP64 fused batches split alike (their own list: no shortened edge tiles)
     p64 = abi.make_params(abi.FP64, limiter=4, tile_rows=32, waves_per_block=4)
-    assert abi.plan_tiles(p64, doms, lanes=True)[1] == 0
+    t64, s64 = abi.plan_tiles(p64, doms, lanes=True)
+    b64 = [i for i in range(1, len(t64)) if t64[i, 0] != t64[i - 1, 0]]
+    assert s64 in b64 and abs(2 * s64 - len(t64)) == min(abs(2 * b - len(t64)) for b in b64)
     # the split moves no tile: the same tiles in both plans
     same = abi.plan_tiles(p, doms, lanes=False)[0]
     key = lambda a: sorted(map(tuple, a.tolist()))  # noqa: E731
