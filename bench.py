@@ -145,7 +145,7 @@ def run_ensemble(abi, ic, spg, tiles=None):
     tkw = dict(TIME_KW)
     tkw["t_final"] = T_ENS_BENCH
     e.time_init(**tkw)
-    tuned = tuple(tiles) if tiles else e.autotune(steps=2)
+    tuned = tuple(tiles) if tiles else e.autotune(steps=4)
     e.time_init(**tkw)
     sync()
     t0 = time.perf_counter()
@@ -331,7 +331,7 @@ def run_friction_batch(abi, ic, spg, t_final=2.0, n_inst=64, tiles=None):
     tkw = dict(TIME_KW)
     tkw["t_final"] = t_final
     e.time_init(**tkw)
-    tuned = tuple(tiles) if tiles else e.autotune(steps=2)
+    tuned = tuple(tiles) if tiles else e.autotune(steps=4)
     e.time_init(**tkw)
     sync()
     t0 = time.perf_counter()
