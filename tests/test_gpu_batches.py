@@ -323,14 +323,19 @@ def test_two_handles_on_two_threads_equal_one_thread():
             assert np.array_equal(qa.view(np.uint64), qb.view(np.uint64))
 
 
-def test_fp32_batch_in_pieces_equals_one_run():
-    """An FP32 batch runs in two lanes (two halves of its instances, their
-    step graphs on two streams at once), its tile list rebuilt and split
-    again as instances finish.  Stopping it at accepted-step limits, retuning
-    its tiles in between (the autotuner times both lanes at once) and going
-    on with other graph lengths ends, bit for bit, where one uninterrupted run
-    ends; two members run alone end there too."""
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_batch_in_pieces_equals_one_run(prec):
+    """A batch runs in two lanes (two halves of its instances, their step
+    graphs on two streams at once), its tile list rebuilt and split again as
+    instances finish (in FP64 each lane's ghost-fill launch covers its own
+    instances, so a lane graph is captured again when the split moves).
+    Stopping it at accepted-step limits, retuning its tiles in between (the
+    autotuner times both lanes at once) and going on with other graph lengths
+    ends, bit for bit, where one uninterrupted run ends; two members run alone
+    end there too."""
     from paraclaws_amd import abi, ic
+
+    precision = abi.FP32 if prec == "fp32" else abi.FP64
 
     sizes = [(40, 40, 2.0), (70, 50, 3.5), (120, 90, 1.7), (33, 150, 6.0), (200, 64, 2.5),
              (90, 90, 4.0)]
@@ -339,7 +344,7 @@ def test_fp32_batch_in_pieces_equals_one_run():
 
     def engine(sel):
         doms = [abi.Domain(sizes[k][0], sizes[k][1], bc=(4, 3, 2, 4)) for k in sel]
-        e = abi.Engine(doms, precision=abi.FP32, limiter=4, tile_rows=16, waves_per_block=2)
+        e = abi.Engine(doms, precision=precision, limiter=4, tile_rows=16, waves_per_block=2)
         for j, k in enumerate(sel):
             nx, ny, r = sizes[k]
             e.set_state(j, ic.dam_break(nx, ny, h_in=r))
@@ -356,11 +361,11 @@ def test_fp32_batch_in_pieces_equals_one_run():
         return out
 
     everyone = list(range(len(sizes)))
-    _say("FP32 batch of six, one run")
+    _say(f"{prec} batch of six, one run")
     with engine(everyone) as e:
         e.run(steps_per_graph=8)
         whole = result(e, len(sizes))
-    _say("FP32 batch of six, in pieces with a retune")
+    _say(f"{prec} batch of six, in pieces with a retune")
     with engine(everyone) as e:
         e.run(steps_per_graph=8, max_accepted=10)
         e.autotune(steps=2)
