@@ -257,7 +257,10 @@ int pcw_stream_bandwidth(int64_t plane_bytes, int32_t reps, double* gbs_best, do
 int64_t pcw_plan_tiles(const pcw_params* p, const pcw_domain* d, int32_t n, int32_t lanes,
                        int32_t* out, int64_t cap, int32_t* split);
 /* Autotuner: times candidate (tile_rows, waves_per_block) on the handle's
- * current state and keeps the fastest; writes the choice. */
+ * current state (`steps` attempts each, a batch's two lanes at once) and
+ * keeps the fastest; for a batch, the tallest height within 1 % of the
+ * fastest, at that height's fastest wave count.  Writes the choice.  No
+ * counterpart in the reference (the reference has no tiles). */
 int pcw_autotune(pcw_handle* h, int32_t steps, int32_t* tile_rows,
                  int32_t* waves_per_block);
 
